@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""AccJerk single-evaluation benchmark on one MI355X.
+
+Workload (the single-GPU kernel configuration the metric is quoted on):
+Plummer sphere, N = 65536, seed 1, Salpeter masses (dN/dm ~ m^-2.35 on
+[0.1, 100], normalised to total mass 1), softening eps = 4/N (per-particle
+eps2 field = eps^2/2), Henon units.  One "step" is one AccJerk evaluation of
+all N^2 (i, j) pairs (self pairs included and masked, as the reference
+counts them) in the FP32-compensated mode, inputs resident in HBM.
+
+Reported:
+  value       N^2 * steps / elapsed, in Ginteractions/s
+  roofline    42 flop per interaction (the reference's count,
+              tupan/lib/src/acc_jerk_kernel_common.h:56) / the pair-engine
+              kernel's mean duration, measured live with HIP events on the
+              launch stream; bound = vector ALU (FP32), peak 157.3 TFLOP/s
+  accuracy    per-particle |a - a64|/|a64| and |j - j64|/|j64| (max, rms)
+              against the FP64 kernel on the same inputs
+  fp64        throughput of the FP64 kernel on the same workload
+  cpu_baseline  the reference's C AccJerk (oracle/_ref, or our bit-identical
+              C restatement when _ref is absent), single-threaded, on a
+              sample of i-rows x all N j-particles
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Acc+Jrk Ginteractions/s at N=64K, FP32-comp vs FP64 at <=1e-6 "
+          "rel err; %peak")
+FLOPS_PER_PAIR = 42
+PEAK_FP32_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=30)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=65536)
+    p.add_argument("--mode", default="fp32c", choices=["fp32c", "fp64"])
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="target CPU work for the cpu_baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-fp64", action="store_true")
+    return p.parse_args()
+
+
+def make_workload(n, seed=1):
+    from tupan_amd import kernels
+    from tupan_amd.ics import PowerLaw, make_plummer
+    phi = lambda ps: kernels.compute("phi", ps, ps, mode="fp64")[0]
+    return make_plummer(n, 4.0 / n, imf=PowerLaw(2.35, 0.1, 100.0), seed=seed,
+                        phi_fn=phi)
+
+
+def rel_err(got, ref):
+    out = {}
+    for name, sl in (("acc", slice(0, 3)), ("jerk", slice(3, 6))):
+        g = np.stack([x.cpu().numpy() for x in got[sl]])
+        r = np.stack([x.cpu().numpy() for x in ref[sl]])
+        e = np.linalg.norm(g - r, axis=0) / np.linalg.norm(r, axis=0)
+        out[name + "_max"] = float(e.max())
+        out[name + "_rms"] = float(np.sqrt(np.mean(e**2)))
+    return out
+
+
+def time_steps(fn, steps, warmup, torch):
+    from tupan_amd import _lib
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    _lib.take_kernel_times()  # drop warm-up launches
+    _lib.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.set_timing(False)
+    k_ms, k_n = _lib.take_kernel_times()
+    return elapsed, k_ms / max(k_n, 1), k_n
+
+
+def cpu_baseline(ps, n, target_s):
+    import oracle
+    kind = "reference" if oracle.available("float64", "reference") else "port"
+    o = oracle.Oracle("float64", kind)
+    fields = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
+    jd = {f: np.ascontiguousarray(getattr(ps, f)) for f in fields}
+    rng = np.random.RandomState(0)
+
+    def run(rows):
+        idx = np.sort(rng.choice(n, rows, replace=False))
+        idd = {f: v[idx] for f, v in jd.items()}
+        t0 = time.perf_counter()
+        o.call("acc_jerk_kernel", idd, jd)
+        return time.perf_counter() - t0
+
+    probe_rows = 64
+    dt = run(probe_rows)
+    rows = int(min(n, max(probe_rows, probe_rows * target_s / max(dt, 1e-6))))
+    dt = run(rows)
+    rate = rows * n / dt
+    return {"value": rate / 1e9, "unit": "Ginteractions/s", "cores": 1,
+            "kind": kind,
+            "sample": "AccJerk FP64, %d sampled i-rows x all %d j (%.1f s, "
+                      "1 thread, %s)" % (rows, n, dt,
+                                         "reference C tupan/lib/src compiled "
+                                         "-O3 -std=c99" if kind == "reference"
+                                         else "bit-identical C restatement")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from tupan_amd import device, spec
+
+    if not torch.cuda.is_available():
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    print("[bench] building Plummer N=%d workload" % n, file=sys.stderr, flush=True)
+    ps = make_workload(n)
+    d = device.to_device(ps, spec.F8)
+    outs = tuple(torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(6))
+
+    def step():
+        device.compute("acc_jerk", d, d, mode=args.mode, out=outs)
+
+    print("[bench] timing %s x%d" % (args.mode, args.steps), file=sys.stderr, flush=True)
+    elapsed, k_ms, k_n = time_steps(step, args.steps, args.warmup, torch)
+    pairs = float(n) * n
+    value = pairs * args.steps / elapsed / 1e9
+    ms_per_step = 1e3 * elapsed / args.steps
+    achieved = FLOPS_PER_PAIR * pairs / (k_ms * 1e-3) / 1e12
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Ginteractions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.mode == "fp32c" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "AccJerk single evaluation, Plummer N=%d seed 1, "
+                        "Salpeter masses [0.1,100], eps=4/N, %s mode"
+                        % (n, args.mode),
+            "n": n, "mode": args.mode,
+            "interactions_per_step": pairs,
+        },
+        "roofline": {
+            "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None, "kernel_ms": k_ms, "kernel_launches": k_n,
+            "flops_per_interaction": FLOPS_PER_PAIR,
+        },
+    }
+    if args.mode == "fp32c" and not args.no_fp64:
+        ref = tuple(torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(6))
+
+        def step64():
+            device.compute("acc_jerk", d, d, mode="fp64", out=ref)
+
+        print("[bench] fp64 reference + timing", file=sys.stderr, flush=True)
+        e64, k64, _ = time_steps(step64, max(3, args.steps // 3), 1, torch)
+        result["fp64"] = {"value": pairs * max(3, args.steps // 3) / e64 / 1e9,
+                          "unit": "Ginteractions/s", "kernel_ms": k64}
+        result["accuracy"] = rel_err(outs, ref)
+        result["accuracy"]["vs"] = "FP64 GPU kernel, same inputs"
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(ps, n, args.cpu_seconds)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

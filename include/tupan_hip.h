@@ -1,0 +1,153 @@
+/*
+ * tupan_hip.h -- C ABI of libtupan_hip.so, the MI355X (gfx950) build of
+ * Tupan's O(N^2) direct-summation kernel family.
+ *
+ * Plain C: pointers, sizes, ints and doubles only.  Two flavours per kernel:
+ *
+ *   tupan_<kernel>_dev(...)   device pointers, asynchronous on `stream`
+ *                             (a hipStream_t passed as void*; NULL = the
+ *                             default stream).  No copies.
+ *   tupan_<kernel>_host(...)  host pointers, synchronous: copies inputs to
+ *                             HBM, runs, copies outputs back.  This is the
+ *                             call the reference's C backend makes
+ *                             (tupan/lib/cffi_backend.py:125-126).
+ *
+ * Inputs are passed as arrays of field pointers, `ip[k]` / `jp[k]`, in the
+ * reference's argument order (tupan/lib/src/libtupan.h), all SoA FP64 of
+ * length ni (i side) or nj (j side):
+ *
+ *   F5  = m, rx, ry, rz, e2
+ *   F8  = F5, vx, vy, vz
+ *   F14 = F8, ax, ay, az, jx, jy, jz
+ *
+ * Outputs `op[k]` likewise, in libtupan.h order.  `e2` holds half the pair
+ * softening (a pair uses ie2 + je2).  Outputs are overwritten (not
+ * accumulated) for i < ni; i and j arrays may alias (ips is jps); ni or
+ * nj = 0 is valid (nj = 0 writes the empty sums).
+ *
+ * mode: TUPAN_MODE_FP64  -- FP64 arithmetic (oracle-grade)
+ *       TUPAN_MODE_FP32C -- FP32 pair arithmetic on FP64-exact position
+ *                           differences with FP64 block accumulation
+ *                           (Phi, Acc, AccJerk, SnapCrackle; Tstep, PNAcc
+ *                           and Sakura always run FP64 and ignore it).
+ *
+ * Every function returns 0 on success or a TUPAN_ERR_* code; the text of the
+ * last error on the calling thread is tupan_last_error().
+ */
+#ifndef TUPAN_HIP_H
+#define TUPAN_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TUPAN_OK 0
+#define TUPAN_ERR_HIP 1           /* HIP runtime failure (no device, OOM) */
+#define TUPAN_ERR_ARG 2           /* invalid argument (mode, NULL, size) */
+#define TUPAN_ERR_NOT_CONVERGED 3 /* Sakura: a Kepler solve hit its cap */
+
+#define TUPAN_MODE_FP64 0
+#define TUPAN_MODE_FP32C 1
+
+/* kernel ids for tupan_set_launch */
+#define TUPAN_K_PHI 0
+#define TUPAN_K_ACC 1
+#define TUPAN_K_ACC_JERK 2
+#define TUPAN_K_SNAP_CRACKLE 3
+#define TUPAN_K_TSTEP 4
+#define TUPAN_K_PNACC 5
+#define TUPAN_K_SAKURA 6
+
+/* ---- runtime ------------------------------------------------------- */
+const char *tupan_version(void);
+const char *tupan_last_error(void);
+int tupan_device_count(int *count);
+/* default mode of the libtupan.h drop-in libraries (initially FP64) */
+int tupan_set_default_mode(int mode);
+int tupan_get_default_mode(void);
+/* Launch shape of the pair engine (autotuner hook): ipl = i-particles per
+ * lane (1, 2 or 4; 0 = automatic), splits = number of j-range splits
+ * (0 = automatic). */
+int tupan_set_launch(int kernel_id, int mode, int ipl, int splits);
+/* Live timing of the pair-engine kernel: while on, every engine launch is
+ * bracketed by HIP events on its stream.  tupan_take_kernel_times waits for
+ * the recorded launches, returns their summed duration (ms) and count, and
+ * clears the record. */
+int tupan_set_timing(int on);
+/* FP32C near field: pairs with r^2 < (m_j/<m>) (c nj^(-1/3))^2 are
+ * evaluated in FP64 (a threshold on the pair force, c mean interparticle
+ * spacings of a unit-size system for a mean-mass j).  c = 0 turns it off;
+ * default 2. */
+int tupan_set_near_field(double c);
+/* Sakura: Kepler solves a pair may spend (default 2^20) before it is
+ * reported as not converged. */
+int tupan_set_sakura_budget(int solves);
+int tupan_take_kernel_times(double *total_ms, int *count);
+
+/* ---- Phi: replaces phi_kernel, tupan/lib/src/libtupan.h:2-15 ---------
+ * ip, jp: F5.  op: phi. */
+int tupan_phi_dev(int mode, size_t ni, const double *const *ip, size_t nj,
+                  const double *const *jp, double *const *op, void *stream);
+int tupan_phi_host(int mode, size_t ni, const double *const *ip, size_t nj,
+                   const double *const *jp, double *const *op);
+
+/* ---- Acc: replaces acc_kernel, libtupan.h:17-32 ----------------------
+ * ip, jp: F5.  op: ax, ay, az. */
+int tupan_acc_dev(int mode, size_t ni, const double *const *ip, size_t nj,
+                  const double *const *jp, double *const *op, void *stream);
+int tupan_acc_host(int mode, size_t ni, const double *const *ip, size_t nj,
+                   const double *const *jp, double *const *op);
+
+/* ---- AccJerk: replaces acc_jerk_kernel, libtupan.h:34-58 -------------
+ * ip, jp: F8.  op: ax, ay, az, jx, jy, jz. */
+int tupan_acc_jerk_dev(int mode, size_t ni, const double *const *ip,
+                       size_t nj, const double *const *jp, double *const *op,
+                       void *stream);
+int tupan_acc_jerk_host(int mode, size_t ni, const double *const *ip,
+                        size_t nj, const double *const *jp,
+                        double *const *op);
+
+/* ---- SnapCrackle: replaces snap_crackle_kernel, libtupan.h:60-96 -----
+ * ip, jp: F14.  op: sx, sy, sz, cx, cy, cz. */
+int tupan_snap_crackle_dev(int mode, size_t ni, const double *const *ip,
+                           size_t nj, const double *const *jp,
+                           double *const *op, void *stream);
+int tupan_snap_crackle_host(int mode, size_t ni, const double *const *ip,
+                            size_t nj, const double *const *jp,
+                            double *const *op);
+
+/* ---- Tstep: replaces tstep_kernel, libtupan.h:98-119 -----------------
+ * ip, jp: F8.  op: dt_a (-> tstep), dt_b (-> tstepij). */
+int tupan_tstep_dev(size_t ni, const double *const *ip, size_t nj,
+                    const double *const *jp, double eta, double *const *op,
+                    void *stream);
+int tupan_tstep_host(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double eta, double *const *op);
+
+/* ---- PNAcc: replaces pnacc_kernel, libtupan.h:121-150 ----------------
+ * ip, jp: F8.  inv[0..6] = 1/c^1 .. 1/c^7.  op: pnax, pnay, pnaz. */
+int tupan_pnacc_dev(size_t ni, const double *const *ip, size_t nj,
+                    const double *const *jp, unsigned order,
+                    const double *inv, double *const *op, void *stream);
+int tupan_pnacc_host(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, unsigned order,
+                     const double *inv, double *const *op);
+
+/* ---- Sakura: replaces sakura_kernel, libtupan.h:203-229 --------------
+ * ip, jp: F8.  op: drx, dry, drz, dvx, dvy, dvz.  d_status (device int,
+ * may be NULL) is OR-ed with 1 when a pair's Kepler sub-stepping hit the
+ * device cap (the reference would loop without bound); the host variant
+ * then returns TUPAN_ERR_NOT_CONVERGED with the results still written. */
+int tupan_sakura_dev(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, int flag,
+                     double *const *op, int *d_status, void *stream);
+int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double dt, int flag,
+                      double *const *op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TUPAN_HIP_H */

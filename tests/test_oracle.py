@@ -1,0 +1,296 @@
+"""Pins for the CPU checker (oracle/).
+
+1. Bit-identity with the golden fixtures produced by the reference's own C
+   kernels (all 127 cases, FP64 and FP32).
+2. Bit-identity with the compiled reference on fresh random inputs, when
+   oracle/_ref was built here.
+3. Mathematics that fixes the kernels independently of any code: Plummer
+   two-body potential, acc = -grad phi, jerk = d(acc)/dt, snap and crackle
+   as higher time derivatives, Newton's third law, the r^2 > 0 mask, the
+   Tstep special case r.v = 0, the 1PN Einstein-Infeld-Hoffmann two-body
+   acceleration and exact Kepler propagation for the Sakura two-body solver.
+   Each of these fails for a dropped term, a wrong sign or a swapped i/j.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import Oracle
+
+from conftest import OUT_GROUPS
+
+
+@pytest.fixture(scope="module")
+def o64():
+    return Oracle("float64", "port")
+
+
+@pytest.fixture(scope="module")
+def o32():
+    return Oracle("float32", "port")
+
+
+SYM = {"phi": "phi_kernel", "acc": "acc_kernel", "acc_jerk": "acc_jerk_kernel",
+       "snap_crackle": "snap_crackle_kernel", "tstep": "tstep_kernel",
+       "pnacc": "pnacc_kernel", "sakura": "sakura_kernel"}
+
+
+def test_port_matches_golden_bitwise(golden, o64, o32):
+    assert len(golden.cases) > 100
+    for case in golden.cases:
+        lib = o64 if case["prec"] == "float64" else o32
+        dt = np.float64 if case["prec"] == "float64" else np.float32
+        ip, jp = golden.inputs(case, dt)
+        got = lib.call(SYM[case["kernel"]], ip, jp, *golden.oracle_args(case))
+        ref = golden.outputs(case)
+        assert len(got) == len(ref)
+        for g, r in zip(got, ref):
+            assert g.dtype == dt
+            assert np.array_equal(g, r, equal_nan=True), (case, g[:4], r[:4])
+
+
+@pytest.mark.skipif(not oracle.available("float64", "reference"),
+                    reason="oracle/_ref not built (needs the reference sources)")
+def test_port_matches_reference_on_fresh_inputs():
+    rng = np.random.RandomState(2024)
+    n = 77
+    d = {f: rng.normal(0, 1, n) for f in oracle.F14}
+    d["mass"] = rng.uniform(0.1, 1, n)
+    d["eps2"] = rng.uniform(0, 1e-3, n)
+    for prec in ("float64", "float32"):
+        P, R = Oracle(prec, "port"), Oracle(prec, "reference")
+        dd = {f: v.astype(P.real) for f, v in d.items()}
+        for name, args in (("phi_kernel", ()), ("acc_kernel", ()),
+                           ("acc_jerk_kernel", ()),
+                           ("snap_crackle_kernel", ()),
+                           ("tstep_kernel", (0.03,)),
+                           ("pnacc_kernel", (7,) + tuple(50.0**-k for k in range(1, 8))),
+                           ("sakura_kernel", (1.0 / 64, 2))):
+            a = P.call(name, dd, dd, *args)
+            b = R.call(name, dd, dd, *args)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y, equal_nan=True), (prec, name)
+
+
+def two_body(m1=0.7, m2=0.3, e21=1e-3, e22=2e-3, seed=0):
+    rng = np.random.RandomState(seed)
+    d = {"mass": np.array([m1, m2]), "eps2": np.array([e21, e22])}
+    for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        d[f] = rng.normal(0, 1, 2)
+    return d
+
+
+def test_phi_two_body_plummer(o64):
+    d = two_body()
+    (phi,) = o64.call("phi_kernel", d, d)
+    r2 = sum((d[f][0] - d[f][1])**2 for f in ("rx", "ry", "rz"))
+    s = np.sqrt(r2 + d["eps2"][0] + d["eps2"][1])
+    assert phi[0] == pytest.approx(-d["mass"][1] / s, rel=1e-15)
+    assert phi[1] == pytest.approx(-d["mass"][0] / s, rel=1e-15)
+
+
+def _acc_from_phi_fd(o64, d, h=1e-5):
+    """acc_i = -d phi_i / d x_i, by central differences of the Phi kernel."""
+    n = len(d["mass"])
+    acc = np.zeros((3, n))
+    for c, f in enumerate(("rx", "ry", "rz")):
+        for i in range(n):
+            ip = {k: v[i:i + 1].copy() for k, v in d.items()}
+            ip[f] = ip[f] + h
+            pp = o64.call("phi_kernel", ip, d)[0][0]
+            ip[f] = ip[f] - 2 * h
+            pm = o64.call("phi_kernel", ip, d)[0][0]
+            acc[c, i] = -(pp - pm) / (2 * h)
+    return acc
+
+
+def cluster(n=12, seed=3, soft=1e-2):
+    rng = np.random.RandomState(seed)
+    d = {"mass": rng.uniform(0.5, 1.5, n) / n,
+         "eps2": np.full(n, soft / 2)}
+    for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        d[f] = rng.normal(0, 1, n)
+    return d
+
+
+def test_acc_is_minus_grad_phi(o64):
+    d = cluster()
+    acc = np.array(o64.call("acc_kernel", d, d))
+    fd = _acc_from_phi_fd(o64, d)
+    assert np.max(np.abs(acc - fd)) < 1e-7 * np.max(np.abs(acc))
+
+
+def test_newton_third_law(o64):
+    d = cluster(n=40, seed=5)
+    ax, ay, az = o64.call("acc_kernel", d, d)
+    for a in (ax, ay, az):
+        assert abs(np.sum(d["mass"] * a)) < 1e-14 * np.sum(np.abs(d["mass"] * a))
+
+
+def _shifted(d, t, derivs):
+    """Particles moved along x(t) = x + v t + a t^2/2 + j t^3/6."""
+    e = {k: v.copy() for k, v in d.items()}
+    for c in "xyz":
+        e["r" + c] = d["r" + c] + d["v" + c] * t
+        e["v" + c] = d["v" + c].copy()
+        if derivs:
+            a, j = d["a" + c], d["j" + c]
+            e["r" + c] = e["r" + c] + a * t**2 / 2 + j * t**3 / 6
+            e["v" + c] = e["v" + c] + a * t + j * t**2 / 2
+    return e
+
+
+def test_jerk_is_time_derivative_of_acc(o64):
+    d = cluster(n=16, seed=7)
+    aj = np.array(o64.call("acc_jerk_kernel", d, d))
+    h = 1e-5
+    ap = np.array(o64.call("acc_kernel", _shifted(d, h, False), _shifted(d, h, False)))
+    am = np.array(o64.call("acc_kernel", _shifted(d, -h, False), _shifted(d, -h, False)))
+    fd = (ap - am) / (2 * h)
+    assert np.max(np.abs(aj[:3] - (ap + am) / 2)) < 1e-8 * np.max(np.abs(aj[:3]))
+    assert np.max(np.abs(aj[3:] - fd)) < 1e-7 * np.max(np.abs(aj[3:]))
+
+
+def test_snap_crackle_are_higher_time_derivatives(o64):
+    """With (a, j) = the acc/jerk of the system, snap = d2a/dt2 and
+    crackle = d3a/dt3 along the Taylor trajectory; the kernel's A = a_i-a_j,
+    J = j_i-j_j inputs make the trajectory's own derivatives exact to O(t^4)
+    only, so compare the t->0 finite differences of the acc kernel."""
+    d = cluster(n=10, seed=9, soft=0.05)
+    aj = o64.call("acc_jerk_kernel", d, d)
+    for f, v in zip(("ax", "ay", "az", "jx", "jy", "jz"), aj):
+        d[f] = v
+    sc = np.array(o64.call("snap_crackle_kernel", d, d))
+    h = 2e-3
+    acc = {}
+    for k in (-2, -1, 0, 1, 2):
+        e = _shifted(d, k * h, True)
+        acc[k] = np.array(o64.call("acc_kernel", e, e))
+    snap_fd = (acc[1] - 2 * acc[0] + acc[-1]) / h**2
+    crackle_fd = (acc[2] - 2 * acc[1] + 2 * acc[-1] - acc[-2]) / (2 * h**3)
+    assert np.max(np.abs(sc[:3] - snap_fd)) < 1e-4 * np.max(np.abs(sc[:3]))
+    assert np.max(np.abs(sc[3:] - crackle_fd)) < 1e-3 * np.max(np.abs(sc[3:]))
+
+
+def test_mask_drops_coincident_pairs(o64):
+    d = cluster(n=6, seed=11)
+    # particle 5 sits on particle 0 with a different velocity: pair (0, 5)
+    # must vanish (unsoftened r^2 == 0), even though e2 > 0
+    for f in ("rx", "ry", "rz"):
+        d[f][5] = d[f][0]
+    keep = [0, 1, 2, 3, 4]
+    sub = {k: v[keep] for k, v in d.items()}
+    ip = {k: v[:1] for k, v in d.items()}
+    full = o64.call("acc_jerk_kernel", ip, d)
+    part = o64.call("acc_jerk_kernel", ip, sub)
+    for a, b in zip(full, part):
+        assert a[0] == b[0]
+
+
+def test_tstep_special_case_radial_velocity_zero(o64):
+    """For r.v = 0 the time-derivative correction vanishes: w^2 =
+    (v^2 + 2 (m1+m2)/s) / s^2 and dt = eta / sqrt(1 + w^2)."""
+    d = {"mass": np.array([0.6, 0.4]), "eps2": np.array([1e-3, 3e-3]),
+         "rx": np.array([0.0, 1.5]), "ry": np.zeros(2), "rz": np.zeros(2),
+         "vx": np.zeros(2), "vy": np.array([0.3, -0.2]), "vz": np.array([0.1, 0.0])}
+    eta = 0.05
+    dta, dtb = o64.call("tstep_kernel", d, d, eta)
+    s2 = 1.5**2 + 4e-3
+    v2 = 0.5**2 + 0.1**2
+    w2 = (v2 + 2 * 1.0 / np.sqrt(s2)) / s2
+    assert dta[0] == pytest.approx(eta / np.sqrt(1 + w2), rel=1e-14)
+    assert dtb[0] == pytest.approx(dta[0], rel=1e-15)
+
+
+def test_pnacc_1pn_is_eih(o64):
+    """Order 2 (1PN) against the Einstein-Infeld-Hoffmann two-body
+    acceleration in harmonic coordinates (G = 1)."""
+    d = two_body(e21=0.0, e22=0.0, seed=4)
+    c = 40.0
+    args = (2,) + tuple(c**-k for k in range(1, 8))
+    ax, ay, az = o64.call("pnacc_kernel", d, d, *args)
+    x = np.array([[d["rx"][i], d["ry"][i], d["rz"][i]] for i in range(2)])
+    v = np.array([[d["vx"][i], d["vy"][i], d["vz"][i]] for i in range(2)])
+    m = d["mass"]
+    for i, j in ((0, 1), (1, 0)):
+        r = np.linalg.norm(x[i] - x[j])
+        n = (x[i] - x[j]) / r
+        a = ((5 * m[i] * m[j] / r**3 + 4 * m[j]**2 / r**3
+              + m[j] / r**2 * (1.5 * np.dot(n, v[j])**2 - np.dot(v[i], v[i])
+                               + 4 * np.dot(v[i], v[j]) - 2 * np.dot(v[j], v[j]))) * n
+             + m[j] / r**2 * (4 * np.dot(n, v[i]) - 3 * np.dot(n, v[j])) * (v[i] - v[j]))
+        got = np.array([ax[i], ay[i], az[i]])
+        assert np.allclose(got, a / c**2, rtol=1e-12, atol=1e-16)
+
+
+def kepler_exact(r0, v0, mu, t):
+    """Classical elliptic Kepler propagation (eccentric anomaly, Newton on
+    Kepler's equation) -- independent of the universal-variable solver."""
+    r0n = np.linalg.norm(r0)
+    a = 1.0 / (2.0 / r0n - np.dot(v0, v0) / mu)
+    assert a > 0
+    n = np.sqrt(mu / a**3)
+    ecosE = 1 - r0n / a
+    esinE = np.dot(r0, v0) / np.sqrt(mu * a)
+    e = np.hypot(ecosE, esinE)
+    E0 = np.arctan2(esinE, ecosE)
+    M = E0 - esinE + n * t
+    E = M
+    for _ in range(100):
+        E = E - (E - e * np.sin(E) - M) / (1 - e * np.cos(E))
+    dE = E - E0
+    f = 1 - a / r0n * (1 - np.cos(dE))
+    g = t - (dE - np.sin(dE)) / n
+    r = f * r0 + g * v0
+    rn = np.linalg.norm(r)
+    fd = -np.sqrt(mu * a) * np.sin(dE) / (rn * r0n)
+    gd = 1 - a / rn * (1 - np.cos(dE))
+    return r, fd * r0 + gd * v0
+
+
+@pytest.mark.parametrize("flag", [-1, 1, -2, 2])
+def test_sakura_twobody_matches_exact_kepler(o64, flag):
+    rng = np.random.RandomState(abs(flag) * 10 + (flag > 0))
+    mu = 1.3
+    r0 = rng.normal(0, 1, 3)
+    v0 = rng.normal(0, 1, 3)
+    # bound, eccentric: speed 0.8 of circular, random direction
+    v0 *= 0.8 * np.sqrt(mu / np.linalg.norm(r0)) / np.linalg.norm(v0)
+    dt = 0.37
+    st = np.concatenate([r0, v0])
+    got = o64.twobody_evolve(dt, flag, mu, 0.0, st)
+    if flag == -1:
+        r, v = kepler_exact(r0 - v0 * dt, v0, mu, dt)
+    elif flag == 1:
+        r, v = kepler_exact(r0, v0, mu, dt)
+        r = r - v * dt
+    elif flag == -2:
+        r, v = kepler_exact(r0 - v0 * dt / 2, v0, mu, dt)
+        r = r - v * dt / 2
+    else:
+        r, v = kepler_exact(r0, v0, mu, dt / 2)
+        r = r - v * dt
+        r, v = kepler_exact(r, v, mu, dt / 2)
+    assert np.allclose(got[:3], r, rtol=0, atol=1e-11)
+    assert np.allclose(got[3:], v, rtol=0, atol=1e-11)
+
+
+def test_sakura_identities(o64):
+    st = np.array([0.3, -0.2, 0.5, 0.1, 0.4, -0.3])
+    assert np.array_equal(o64.twobody_evolve(0.2, 0, 1.0, 1e-4, st), st)
+    got = o64.twobody_evolve(0.0, -1, 1.0, 0.0, st)
+    assert np.allclose(got, st, atol=1e-14)
+
+
+def test_sakura_softened_energy_conserved(o64):
+    """The softened two-body energy v^2/2 - m/sqrt(r^2 + e2) is conserved by
+    the Kepler branch to the solver's tolerance (64 * 2^-42 relative)."""
+    m, e2 = 1.0, 1e-4
+    st = np.array([0.5, 0.1, 0.0, 0.2, 1.1, 0.1])
+    E = lambda s: 0.5 * np.dot(s[3:], s[3:]) - m / np.sqrt(np.dot(s[:3], s[:3]) + e2)
+    # flag 2 with the drift undone: K(dt/2) D(-dt) K(dt/2); use flag 1 and
+    # add the drift back to isolate K(dt)
+    dt = 0.05
+    out = o64.twobody_evolve(dt, 1, m, e2, st)
+    out[:3] += out[3:] * dt
+    assert abs(E(out) - E(st)) < 1e-10 * abs(E(st))

@@ -1,0 +1,200 @@
+"""ctypes binding of libtupan_hip.so (include/tupan_hip.h).
+
+The library is built in-tree by ``tupan_amd.build``.  There is no fallback:
+if the shared object is missing, or the HIP runtime reports no device,
+every call raises ``TupanError``.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from . import spec
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtupan_hip.so")
+
+MODE_FP64 = 0
+MODE_FP32C = 1
+MODES = {"fp64": MODE_FP64, "fp32c": MODE_FP32C}
+
+OK = 0
+ERR_HIP = 1
+ERR_ARG = 2
+ERR_NOT_CONVERGED = 3
+
+
+class TupanError(RuntimeError):
+    def __init__(self, code, msg):
+        RuntimeError.__init__(self, "tupan_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class NotConverged(TupanError):
+    pass
+
+
+_lock = threading.Lock()
+_lib = None
+
+_P = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_SZ = ctypes.c_size_t
+
+
+def lib():
+    """Load (once) and return the CDLL; raise TupanError if not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise TupanError(-1, "HIP library not built: %s (run "
+                             "python -m tupan_amd.build)" % LIB_PATH)
+        # torch-ROCm bundles its own HIP runtime under the same soname as
+        # /opt/rocm's.  Whichever loads first serves the whole process, and
+        # torch cannot see the GPU through a runtime other than its own, so
+        # let torch load its runtime before ours binds to it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        L.tupan_version.restype = ctypes.c_char_p
+        L.tupan_last_error.restype = ctypes.c_char_p
+        L.tupan_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.tupan_set_default_mode.argtypes = [ctypes.c_int]
+        L.tupan_set_launch.argtypes = [ctypes.c_int] * 4
+        L.tupan_set_timing.argtypes = [ctypes.c_int]
+        L.tupan_set_near_field.argtypes = [ctypes.c_double]
+        L.tupan_set_sakura_budget.argtypes = [ctypes.c_int]
+        L.tupan_take_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_int)]
+        for name in ("phi", "acc", "acc_jerk", "snap_crackle"):
+            getattr(L, "tupan_%s_dev" % name).argtypes = [
+                ctypes.c_int, _SZ, _PP, _SZ, _PP, _PP, _P]
+            getattr(L, "tupan_%s_host" % name).argtypes = [
+                ctypes.c_int, _SZ, _PP, _SZ, _PP, _PP]
+        L.tupan_tstep_dev.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_double,
+                                      _PP, _P]
+        L.tupan_tstep_host.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_double,
+                                       _PP]
+        if hasattr(L, "tupan_pnacc_dev"):
+            L.tupan_pnacc_dev.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_uint,
+                                          _P, _PP, _P]
+            L.tupan_pnacc_host.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_uint,
+                                           _P, _PP]
+        if hasattr(L, "tupan_sakura_dev"):
+            L.tupan_sakura_dev.argtypes = [_SZ, _PP, _SZ, _PP,
+                                           ctypes.c_double, ctypes.c_int,
+                                           _PP, _P, _P]
+            L.tupan_sakura_host.argtypes = [_SZ, _PP, _SZ, _PP,
+                                            ctypes.c_double, ctypes.c_int,
+                                            _PP]
+        _lib = L
+        return _lib
+
+
+def check(rc):
+    if rc == OK:
+        return
+    msg = lib().tupan_last_error().decode(errors="replace")
+    if rc == ERR_NOT_CONVERGED:
+        raise NotConverged(rc, msg)
+    raise TupanError(rc, msg)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = lib().tupan_device_count(ctypes.byref(n))
+    return n.value if rc == OK else 0
+
+
+def set_launch(kernel, mode, ipl=0, splits=0):
+    """Autotuner hook: fix the launch shape of one kernel/mode."""
+    check(lib().tupan_set_launch(spec.KERNEL_IDS[kernel], MODES.get(mode, mode),
+                                 int(ipl), int(splits)))
+
+
+def set_near_field(c):
+    """FP32C near-field factor (0 disables the FP64 near-field branch)."""
+    check(lib().tupan_set_near_field(float(c)))
+
+
+def set_sakura_budget(solves):
+    check(lib().tupan_set_sakura_budget(int(solves)))
+
+
+def set_timing(on):
+    check(lib().tupan_set_timing(1 if on else 0))
+
+
+def take_kernel_times():
+    """(total_ms, launches) of the pair-engine kernels timed so far."""
+    ms = ctypes.c_double(0)
+    n = ctypes.c_int(0)
+    check(lib().tupan_take_kernel_times(ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def ptr_array(ptrs):
+    arr = (ctypes.c_void_p * max(1, len(ptrs)))()
+    for k, p in enumerate(ptrs):
+        arr[k] = p
+    return arr
+
+
+def _cargs(name, mode, scalars):
+    """Leading mode and trailing scalar arguments of one kernel."""
+    k = spec.KERNELS[name]
+    pre = [ctypes.c_int(MODES.get(mode, mode))] if k.has_mode else []
+    post = []
+    if name == "tstep":
+        post = [ctypes.c_double(float(scalars[0]))]
+    elif name == "pnacc":
+        order, inv = scalars
+        inv = (ctypes.c_double * 7)(*[float(x) for x in inv])
+        post = [ctypes.c_uint(int(order)), ctypes.cast(inv, _P)]
+        post.append(inv)  # keep alive; stripped before the call
+    elif name == "sakura":
+        post = [ctypes.c_double(float(scalars[0])), ctypes.c_int(int(scalars[1]))]
+    return pre, post
+
+
+def call_host(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=()):
+    """Synchronous host-pointer call (tupan_<name>_host)."""
+    L = lib()
+    pre, post = _cargs(name, mode, scalars)
+    keep = post[2:] if name == "pnacc" else []
+    post = post[:2] if name == "pnacc" else post
+    ia, ja, oa = ptr_array(iptrs), ptr_array(jptrs), ptr_array(optrs)
+    fn = getattr(L, "tupan_%s_host" % name)
+    args = pre + [_SZ(ni), ia, _SZ(nj), ja] + post + [oa]
+    rc = fn(*args)
+    del keep
+    check(rc)
+
+
+def call_dev(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=(), stream=0,
+             status_ptr=None):
+    """Asynchronous device-pointer call (tupan_<name>_dev) on ``stream``."""
+    L = lib()
+    pre, post = _cargs(name, mode, scalars)
+    keep = post[2:] if name == "pnacc" else []
+    post = post[:2] if name == "pnacc" else post
+    ia, ja, oa = ptr_array(iptrs), ptr_array(jptrs), ptr_array(optrs)
+    fn = getattr(L, "tupan_%s_dev" % name)
+    args = pre + [_SZ(ni), ia, _SZ(nj), ja] + post + [oa]
+    if name == "sakura":
+        args.append(_P(status_ptr or 0))
+    args.append(_P(stream or 0))
+    rc = fn(*args)
+    del keep
+    check(rc)
+
+
+def as_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)

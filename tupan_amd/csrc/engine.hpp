@@ -1,0 +1,288 @@
+// engine.hpp -- the O(ni*nj) pair-sum engine for gfx950 (MI355X).
+//
+// One templated kernel runs every member of the family.  A "policy" class
+// supplies the per-kernel pieces:
+//
+//   JRec             packed j-particle record (built once per call by
+//                    pack_kernel from the caller's SoA FP64 arrays)
+//   I                the i-particle state a lane keeps in registers
+//   A32 / A64        per-lane accumulators: A32 is summed over one j-tile,
+//                    then folded into A64 (kTwoLevel); FP64 policies
+//                    accumulate straight into A64
+//   load_i, pair, fold, combine, finish
+//
+// Execution model (CDNA4, wave64):
+//   * a wave owns 64*P consecutive i-particles, P per lane, in VGPRs;
+//   * all 64 lanes walk the same j sequence, so a j record is wave-uniform:
+//     it is fetched with scalar loads (s_load) into SGPRs and used directly
+//     as a VALU operand -- no LDS round trip, no bank conflicts;
+//   * the j range is split S ways across blockIdx.y so that the chip holds
+//     >= 4 waves per SIMD at any N; the S partial sums land in an FP64
+//     scratch array and a finishing kernel adds them in fixed order
+//     (bit-reproducible run to run);
+//   * blocks are 256 threads (4 independent waves); consecutive blocks go
+//     round-robin to the 8 XCDs and all waves of one split read the same j
+//     records, which every XCD's L2 holds (<= 64 B * nj).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace tupan {
+
+enum Field { M = 0, RX, RY, RZ, E2, VX, VY, VZ, AX, AY, AZ, JX, JY, JZ };
+
+struct Soa {
+  const double *f[14];
+};
+struct Out {
+  double *f[6];
+};
+
+constexpr int kWaves = 4;       // waves per block
+constexpr int kBlock = 64 * kWaves;
+constexpr int kTile = 32;       // j per FP32 tile before the FP64 fold
+
+// ---------------------------------------------------------------- math
+// 1/sqrt(x) in FP64: hardware estimate (v_rsq_f64, ~2^-23 relative)
+// refined by one third-order Newton-Householder step:
+//   e = 1 - x y^2,  y <- y (1 + e/2 + 3 e^2/8)  (error ~ e^3).
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double t = x * y;
+  double e = __builtin_fma(-t, y, 1.0);
+  double p = __builtin_fma(0.375, e, 0.5);
+  return __builtin_fma(y * e, p, y);
+}
+
+// 1/sqrt(x) in FP32: v_rsq_f32 (1 ulp).
+__device__ __forceinline__ float rsq32(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+
+// FP32 difference of two FP64 coordinates: exact FP64 subtraction
+// followed by one rounding to FP32.  This is what the hi/lo position split
+// buys -- the pair separation is correct to 1/2 ulp of itself even when
+// |x| >> |xi - xj| -- in two VALU ops (v_add_f64 + v_cvt_f32_f64, both
+// full rate on CDNA4) instead of three FP32 ops on hi/lo pairs.
+__device__ __forceinline__ float dsub32(double a, double b) {
+  return static_cast<float>(a - b);
+}
+
+// 1 if some active lane is missing from the lane mask `far` (the ballot of
+// the lanes whose pair is NOT near).  Two scalar instructions: written out
+// because the compiler otherwise round-trips the uniform boolean through a
+// VGPR (v_cndmask + v_readfirstlane), and taking the ballot of the "far"
+// predicate lets the one v_cmp feed both this mask and the FP32 select.
+// No memory access.
+__device__ __forceinline__ uint32_t near_bit(uint64_t far) {
+  uint32_t b;
+  asm volatile("s_cmp_lg_u64 %1, exec\n\ts_cselect_b32 %0, 1, 0"
+               : "=s"(b)
+               : "s"(far)
+               : "scc");
+  return b;
+}
+
+// ------------------------------------------------------------- engine
+// FP32C policies (kTwoLevel) also receive the caller's FP64 arrays and the
+// j index: a pair they judge "near" is re-evaluated from those in FP64.
+template <class Pol, int P>
+__global__ __launch_bounds__(kBlock) void pair_engine(
+    Soa is, Soa js, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
+    int jchunk, double *__restrict__ part, Out out,
+    typename Pol::Params prm) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int i0 = tile * 64 * P;
+  if (i0 >= ni) return;  // wave-uniform; no block barriers below
+  const int s = blockIdx.y;
+
+  typename Pol::I ist[P];
+  typename Pol::A64 acc[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    int i = i0 + p * 64 + lane;
+    i = i < ni ? i : ni - 1;
+    ist[p] = Pol::load_i(is, i, prm);
+    Pol::zero(acc[p]);
+  }
+
+  const int jb = s * jchunk;
+  const int je = min(nj, jb + jchunk);
+  if constexpr (Pol::kTwoLevel) {
+    // FP32 tiles of kTile j's.  pair32 returns the wave's mask of lanes
+    // whose pair is in the far field (near pairs contribute zero in FP32);
+    // the wave-uniform bit k of near[p] records that some lane of slot p
+    // met j0 + k in its near field, and after the tile those (k, p) are
+    // re-evaluated in FP64 for exactly those lanes (near_pair).  The hot loop carries no FP64
+    // code, so the register budget stays that of the FP32 path.
+    for (int j0 = jb; j0 < je; j0 += kTile) {
+      const int nk = min(kTile, je - j0);
+      typename Pol::A32 a[P];
+      uint32_t near[P];
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        Pol::zero(a[p]);
+        near[p] = 0u;
+      }
+      if (nk == kTile) {
+#pragma unroll 2
+        for (int k = 0; k < kTile; ++k) {
+          const typename Pol::JRec r = jr[j0 + k];
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            near[p] |= near_bit(Pol::pair32(ist[p], r, a[p])) << k;
+          }
+        }
+      } else {
+        for (int k = 0; k < nk; ++k) {
+          const typename Pol::JRec r = jr[j0 + k];
+#pragma unroll
+          for (int p = 0; p < P; ++p) {
+            near[p] |= near_bit(Pol::pair32(ist[p], r, a[p])) << k;
+          }
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        Pol::fold(a[p], acc[p]);
+        uint32_t bits = near[p];
+        while (bits) {
+          const int k = __builtin_ctz(bits);
+          bits &= bits - 1u;
+          Pol::near_pair(ist[p], jr[j0 + k], acc[p], is, js, j0 + k);
+        }
+      }
+    }
+  } else {
+#pragma unroll 2
+    for (int j = jb; j < je; ++j) {
+      const typename Pol::JRec r = jr[j];
+#pragma unroll
+      for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
+    }
+  }
+
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int i = i0 + p * 64 + lane;
+    if (i >= ni) continue;
+    if (gridDim.y == 1) {
+      double v[Pol::kOut];
+      Pol::values(acc[p], v);
+      Pol::finish(v, out, i, prm);
+    } else {
+      double v[Pol::kOut];
+      Pol::values(acc[p], v);
+#pragma unroll
+      for (int k = 0; k < Pol::kOut; ++k)
+        part[((size_t)s * Pol::kOut + k) * ni + i] = v[k];
+    }
+  }
+}
+
+// Adds the S split partials in split order and applies the policy's
+// finishing step (e.g. Tstep's eta / sqrt(1 + w2)).
+template <class Pol>
+__global__ __launch_bounds__(256) void finish_kernel(
+    const double *__restrict__ part, int S, int ni, Out out,
+    typename Pol::Params prm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ni) return;
+  double v[Pol::kOut];
+#pragma unroll
+  for (int k = 0; k < Pol::kOut; ++k) v[k] = part[(size_t)k * ni + i];
+  for (int s = 1; s < S; ++s) {
+    double w[Pol::kOut];
+#pragma unroll
+    for (int k = 0; k < Pol::kOut; ++k)
+      w[k] = part[((size_t)s * Pol::kOut + k) * ni + i];
+    Pol::combine(v, w);
+  }
+  Pol::finish(v, out, i, prm);
+}
+
+// Packs the caller's SoA j fields into the policy's record.  `scale` (may
+// be null) is the near-field factor written by near_scale_kernel.
+template <class Pol>
+__global__ __launch_bounds__(256) void pack_kernel(
+    Soa js, int nj, typename Pol::JRec *__restrict__ jr,
+    const double *__restrict__ scale) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nj) jr[j] = Pol::pack(js, j, scale ? *scale : 0.0);
+}
+
+// Near-field radius for the FP32C policies.  A pair (i, j) is "near" when
+// r_ij^2 < (m_j / <m>) * (c * nj^(-1/3))^2: inside c mean interparticle
+// spacings of a unit-size system, widened by the square root of the
+// j-mass in units of the mean mass, i.e. a fixed threshold on the pair
+// force m_j / r^2.  Writes the factor (c^2 nj^(1/3)) / sum_j m_j so that
+// near2_j = m_j * factor.  One block, fixed reduction order.
+static __global__ __launch_bounds__(1024) void near_scale_kernel(
+    const double *__restrict__ m, int nj, double c, double *__restrict__ out) {
+  __shared__ double part[1024];
+  double s = 0.0;
+  for (int j = threadIdx.x; j < nj; j += 1024) s += m[j];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double msum = part[0];
+    double f = c * c * cbrt((double)nj) / msum;
+    out[0] = (msum > 0.0 && f < 1e300) ? f : 0.0;
+  }
+}
+
+// Shared defaults for policies whose outputs are plain sums.
+template <int N>
+struct SumOut {
+  static constexpr int kOut = N;
+  __device__ static void combine(double *v, const double *w) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += w[k];
+  }
+  template <class Params>
+  __device__ static void finish(const double *v, Out out, int i,
+                                const Params &) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) out.f[k][i] = v[k];
+  }
+};
+
+struct NoParams {};
+
+template <int N>
+struct F64Acc {
+  double v[N];
+};
+template <int N>
+struct F32Acc {
+  float v[N];
+};
+
+template <int N>
+__device__ __forceinline__ void zero_acc(F64Acc<N> &a) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) a.v[k] = 0.0;
+}
+template <int N>
+__device__ __forceinline__ void zero_acc(F32Acc<N> &a) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) a.v[k] = 0.0f;
+}
+template <int N>
+__device__ __forceinline__ void fold_acc(const F32Acc<N> &a, F64Acc<N> &b) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) b.v[k] += static_cast<double>(a.v[k]);
+}
+template <int N>
+__device__ __forceinline__ void values_acc(const F64Acc<N> &a, double *v) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = a.v[k];
+}
+
+}  // namespace tupan

@@ -1,0 +1,535 @@
+// gravity.hip -- Phi, Acc, AccJerk, SnapCrackle and Tstep on the pair engine.
+//
+// Each kernel has an FP64 policy (FP64 arithmetic, hardware rsq refined by
+// one Newton-Householder step) and, except Tstep, an FP32C policy:
+//   * r_ij = x_i - x_j is formed in FP64 and rounded once to FP32
+//     (dsub32), so the separation is accurate to 1/2 ulp of itself;
+//   * the rest of the pair arithmetic is FP32 with v_rsq_f32;
+//   * sums run in FP32 over a tile of kTile j's and are folded into FP64
+//     accumulators after every tile (block-compensated summation);
+//   * near field: a pair inside its near-field radius (r^2 < near2_j, see
+//     near_scale_kernel) contributes nothing in FP32; after each tile the
+//     engine re-evaluates exactly those pairs with the FP64 policy from the
+//     caller's FP64 arrays.  The few largest pair forces of each particle --
+//     the ones whose FP32 rounding would dominate its error when they
+//     nearly cancel -- are thereby exact to FP64.
+//
+// Pair formulas and semantics follow the reference cores:
+//   Phi         tupan/lib/src/phi_kernel_common.h:7-31
+//   Acc         tupan/lib/src/acc_kernel_common.h:7-37
+//   AccJerk     tupan/lib/src/acc_jerk_kernel_common.h:7-55
+//   SnapCrackle tupan/lib/src/snap_crackle_kernel_common.h:7-94
+//   Tstep       tupan/lib/src/tstep_kernel_common.h:7-56 (+ finalisation
+//               tupan/lib/src/tstep_kernel.c:46-47)
+// with the softening of tupan/lib/src/smoothing.h:10-184: s^2 = r^2 + e2_i +
+// e2_j, and a pair contributes nothing unless the *unsoftened* r^2 > 0.
+#include "launch.hpp"
+
+namespace tupan {
+
+// Common shape of the FP32C policies: P64 is the FP64 policy used for the
+// near field; the derived policy supplies JRec, I, pack, load_i and body32.
+template <class Derived, class P64, int N>
+struct F32cBase : SumOut<N> {
+  static constexpr bool kTwoLevel = true;
+  using Params = NoParams;
+  using A64 = F64Acc<N>;
+  using A32 = F32Acc<N>;
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void zero(A32 &a) { zero_acc(a); }
+  __device__ static void fold(const A32 &a, A64 &b) { fold_acc(a, b); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+
+  // Separation and the near-field test.  r^2 uses explicit FMAs so that
+  // pair32 and near_pair take bit-identical decisions.  near2 >= FLT_MIN,
+  // so coincident particles (r^2 = 0, masked by the reference) always take
+  // the FP64 path, which applies the mask (as does a NaN separation).
+  template <class I, class JRec>
+  __device__ static bool near_test(const I &q, const JRec &r, float &rx,
+                                   float &ry, float &rz, float &r2) {
+    rx = dsub32(q.x, r.x);
+    ry = dsub32(q.y, r.y);
+    rz = dsub32(q.z, r.z);
+    r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+    return !(r2 >= r.near2);
+  }
+  // FP32 contribution of a far pair; near pairs contribute zero here.
+  // Returns the wave's mask of far lanes.
+  template <class I, class JRec>
+  __device__ static uint64_t pair32(const I &q, const JRec &r, A32 &a) {
+    float rx, ry, rz, r2;
+    const bool far = !near_test(q, r, rx, ry, rz, r2);
+    Derived::body32(q, r, rx, ry, rz, r2, far, a);
+    return __builtin_amdgcn_ballot_w64(far);
+  }
+  // FP64 contribution of a near pair, from the caller's FP64 arrays.
+  template <class I, class JRec>
+  __device__ static void near_pair(const I &q, const JRec &r, A64 &a64,
+                                   const Soa &is, const Soa &js, int j) {
+    float rx, ry, rz, r2;
+    if (near_test(q, r, rx, ry, rz, r2)) {
+      typename P64::Params p64{};
+      P64::pair(P64::load_i(is, q.i, p64), P64::pack(js, j, 0.0), a64, p64);
+    }
+  }
+};
+
+__device__ __forceinline__ float near2_of(double m, double sc) {
+  return fmaxf((float)(m * sc), 1.17549435e-38f);  // >= FLT_MIN
+}
+
+// ================================================================ Phi
+struct PhiF64 : SumOut<1> {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 5, kMaxIpl = 4;
+  using Params = NoParams;
+  struct JRec { double m, x, y, z, e2; };
+  struct I { double x, y, z, e2; };
+  using A64 = F64Acc<1>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j], s.f[RX][j], s.f[RY][j], s.f[RZ][j], s.f[E2][j]};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i]};
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = r2 > 0.0 ? y : 0.0;
+    a.v[0] -= r.m * y;
+  }
+};
+
+// j record / i state shared by the Phi and Acc FP32C policies
+struct Pos32cJ { double x, y, z; float m, e2, near2, pad; };
+struct Pos32cI { double x, y, z; float e2; int i; };
+
+__device__ __forceinline__ Pos32cJ pack_pos32c(const Soa &s, int j,
+                                               double sc) {
+  return {s.f[RX][j], s.f[RY][j], s.f[RZ][j], (float)s.f[M][j],
+          (float)s.f[E2][j], near2_of(s.f[M][j], sc), 0.0f};
+}
+__device__ __forceinline__ Pos32cI load_pos32c(const Soa &s, int i) {
+  return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], (float)s.f[E2][i], i};
+}
+
+struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
+  static constexpr int kFields = 5, kMaxIpl = 4;
+  using JRec = Pos32cJ;
+  using I = Pos32cI;
+  __device__ static JRec pack(const Soa &s, int j, double sc) {
+    return pack_pos32c(s, j, sc);
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return load_pos32c(s, i);
+  }
+  __device__ static void body32(const I &q, const JRec &r, float, float,
+                                float, float r2, bool far, A32 &a) {
+    float y = rsq32(r2 + (q.e2 + r.e2));
+    y = far ? y : 0.0f;
+    a.v[0] -= r.m * y;
+  }
+};
+
+// ================================================================ Acc
+struct AccF64 : SumOut<3> {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 5, kMaxIpl = 4;
+  using Params = NoParams;
+  using JRec = PhiF64::JRec;
+  using I = PhiF64::I;
+  using A64 = F64Acc<3>;
+  __device__ static JRec pack(const Soa &s, int j, double sc) {
+    return PhiF64::pack(s, j, sc);
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &p) {
+    return PhiF64::load_i(s, i, p);
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = r2 > 0.0 ? y : 0.0;
+    double k = r.m * y * (y * y);
+    a.v[0] -= k * rx;
+    a.v[1] -= k * ry;
+    a.v[2] -= k * rz;
+  }
+};
+
+struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
+  static constexpr int kFields = 5, kMaxIpl = 4;
+  using JRec = Pos32cJ;
+  using I = Pos32cI;
+  __device__ static JRec pack(const Soa &s, int j, double sc) {
+    return pack_pos32c(s, j, sc);
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return load_pos32c(s, i);
+  }
+  __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
+                                float rz, float r2, bool far, A32 &a) {
+    float y = rsq32(r2 + (q.e2 + r.e2));
+    y = far ? y : 0.0f;
+    float k = r.m * y * (y * y);
+    a.v[0] -= k * rx;
+    a.v[1] -= k * ry;
+    a.v[2] -= k * rz;
+  }
+};
+
+// ============================================================ AccJerk
+struct AccJerkF64 : SumOut<6> {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 8, kMaxIpl = 4;
+  using Params = NoParams;
+  struct JRec { double m, x, y, z, e2, vx, vy, vz; };
+  struct I { double x, y, z, e2, vx, vy, vz; };
+  using A64 = F64Acc<6>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
+            s.f[E2][j], s.f[VX][j], s.f[VY][j], s.f[VZ][j]};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i],
+            s.f[VX][i], s.f[VY][i], s.f[VZ][i]};
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double vx = q.vx - r.vx, vy = q.vy - r.vy, vz = q.vz - r.vz;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double rv = rx * vx + ry * vy + rz * vz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = r2 > 0.0 ? y : 0.0;
+    double y2 = y * y;
+    double k = r.m * y * y2;
+    double alpha = 3.0 * rv * y2;
+    vx -= alpha * rx;
+    vy -= alpha * ry;
+    vz -= alpha * rz;
+    a.v[0] -= k * rx;
+    a.v[1] -= k * ry;
+    a.v[2] -= k * rz;
+    a.v[3] -= k * vx;
+    a.v[4] -= k * vy;
+    a.v[5] -= k * vz;
+  }
+};
+
+struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
+  static constexpr int kFields = 8, kMaxIpl = 4;
+  // velocities stay FP64 like positions: rounding v_i to FP32 would add the
+  // same error to every pair of particle i, and that coherent term
+  // (dv_i * sum_j m_j/s^3) does not average out in the jerk
+  struct JRec { double x, y, z, vx, vy, vz; float m, e2, near2, pad; };
+  struct I { double x, y, z, vx, vy, vz; float e2; int i; };
+  __device__ static JRec pack(const Soa &s, int j, double sc) {
+    return {s.f[RX][j],       s.f[RY][j],        s.f[RZ][j],
+            s.f[VX][j],       s.f[VY][j],        s.f[VZ][j],
+            (float)s.f[M][j], (float)s.f[E2][j], near2_of(s.f[M][j], sc),
+            0.0f};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i],        s.f[VX][i],
+            s.f[VY][i], s.f[VZ][i], (float)s.f[E2][i], i};
+  }
+  __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
+                                float rz, float r2, bool far, A32 &a) {
+    float vx = dsub32(q.vx, r.vx), vy = dsub32(q.vy, r.vy),
+          vz = dsub32(q.vz, r.vz);
+    float rv = rx * vx + ry * vy + rz * vz;
+    float y = rsq32(r2 + (q.e2 + r.e2));
+    y = far ? y : 0.0f;
+    float y2 = y * y;
+    float k = r.m * y * y2;
+    float alpha = 3.0f * rv * y2;
+    vx -= alpha * rx;
+    vy -= alpha * ry;
+    vz -= alpha * rz;
+    a.v[0] -= k * rx;
+    a.v[1] -= k * ry;
+    a.v[2] -= k * rz;
+    a.v[3] -= k * vx;
+    a.v[4] -= k * vy;
+    a.v[5] -= k * vz;
+  }
+};
+
+// ========================================================= SnapCrackle
+// Snap and crackle of the softened pair force from the relative state
+// (r, v, A = a_i - a_j, J = j_i - j_j): the second and third time
+// derivatives of -m r / s^3 via the alpha, beta, gamma recurrences
+// (snap_crackle_kernel_common.h:63-93).
+template <class T>
+__device__ __forceinline__ void snap_crackle_pair(
+    T rx, T ry, T rz, T vx, T vy, T vz, T Ax, T Ay, T Az, T Jx, T Jy, T Jz,
+    T y, T m, T *acc) {
+  T r2y = y * y;  // 1/s^2
+  T rv = rx * vx + ry * vy + rz * vz;
+  T v2 = vx * vx + vy * vy + vz * vz;
+  T rJ = rx * Jx + ry * Jy + rz * Jz;
+  T rA = rx * Ax + ry * Ay + rz * Az;
+  T vA = vx * Ax + vy * Ay + vz * Az;
+  T alpha = rv * r2y;
+  T alpha2 = alpha * alpha;
+  T beta = T(3) * ((v2 + rA) * r2y + alpha2);
+  T gamma = (T(3) * vA + rJ) * r2y + alpha * (beta - T(4) * alpha2);
+  alpha *= T(3);
+  gamma *= T(3);
+  vx -= alpha * rx;
+  vy -= alpha * ry;
+  vz -= alpha * rz;
+  T two_alpha = T(2) * alpha;
+  Ax -= two_alpha * vx + beta * rx;
+  Ay -= two_alpha * vy + beta * ry;
+  Az -= two_alpha * vz + beta * rz;
+  alpha *= T(3);
+  beta *= T(3);
+  Jx -= alpha * Ax + beta * vx + gamma * rx;
+  Jy -= alpha * Ay + beta * vy + gamma * ry;
+  Jz -= alpha * Az + beta * vz + gamma * rz;
+  T k = m * y * r2y;
+  acc[0] -= k * Ax;
+  acc[1] -= k * Ay;
+  acc[2] -= k * Az;
+  acc[3] -= k * Jx;
+  acc[4] -= k * Jy;
+  acc[5] -= k * Jz;
+}
+
+struct SnapCrackleF64 : SumOut<6> {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 14, kMaxIpl = 2;
+  using Params = NoParams;
+  struct JRec { double m, x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
+  struct I { double x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
+  using A64 = F64Acc<6>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j], s.f[E2][j],
+            s.f[VX][j], s.f[VY][j], s.f[VZ][j], s.f[AX][j], s.f[AY][j],
+            s.f[AZ][j], s.f[JX][j], s.f[JY][j], s.f[JZ][j]};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i], s.f[VX][i],
+            s.f[VY][i], s.f[VZ][i], s.f[AX][i], s.f[AY][i], s.f[AZ][i],
+            s.f[JX][i], s.f[JY][i], s.f[JZ][i]};
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = r2 > 0.0 ? y : 0.0;
+    snap_crackle_pair<double>(rx, ry, rz, q.vx - r.vx, q.vy - r.vy,
+                              q.vz - r.vz, q.ax - r.ax, q.ay - r.ay,
+                              q.az - r.az, q.jx - r.jx, q.jy - r.jy,
+                              q.jz - r.jz, y, r.m, a.v);
+  }
+};
+
+struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
+  static constexpr int kFields = 14, kMaxIpl = 2;
+  // all relative quantities are formed in FP64 and rounded once (see
+  // AccJerkF32c)
+  struct JRec {
+    double x, y, z, vx, vy, vz, ax, ay, az, jx, jy, jz;
+    float m, e2, near2, pad;
+  };
+  struct I {
+    double x, y, z, vx, vy, vz, ax, ay, az, jx, jy, jz;
+    float e2;
+    int i;
+  };
+  __device__ static JRec pack(const Soa &s, int j, double sc) {
+    return {s.f[RX][j],       s.f[RY][j],        s.f[RZ][j], s.f[VX][j],
+            s.f[VY][j],       s.f[VZ][j],        s.f[AX][j], s.f[AY][j],
+            s.f[AZ][j],       s.f[JX][j],        s.f[JY][j], s.f[JZ][j],
+            (float)s.f[M][j], (float)s.f[E2][j], near2_of(s.f[M][j], sc),
+            0.0f};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i],
+            s.f[VY][i], s.f[VZ][i], s.f[AX][i], s.f[AY][i],
+            s.f[AZ][i], s.f[JX][i], s.f[JY][i], s.f[JZ][i],
+            (float)s.f[E2][i], i};
+  }
+  __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
+                                float rz, float r2, bool far, A32 &a) {
+    float y = rsq32(r2 + (q.e2 + r.e2));
+    y = far ? y : 0.0f;
+    snap_crackle_pair<float>(
+        rx, ry, rz, dsub32(q.vx, r.vx), dsub32(q.vy, r.vy),
+        dsub32(q.vz, r.vz), dsub32(q.ax, r.ax), dsub32(q.ay, r.ay),
+        dsub32(q.az, r.az), dsub32(q.jx, r.jx), dsub32(q.jy, r.jy),
+        dsub32(q.jz, r.jz), y, r.m, a.v);
+  }
+};
+
+// ============================================================== Tstep
+// Per pair w^2 = (v^2 + 2 phi)/s^2 with phi = (m_i+m_j)/s, corrected by the
+// time derivative term -gamma (r.v) with gamma = (w^2 + 2 phi/s^2)/s^2 *
+// eta/sqrt(w^2); the i-particle keeps sum_j w^2 and max_j w^2 and returns
+// eta/sqrt(1 + .) of each (tstep_kernel_common.h:41-55, tstep_kernel.c:46-47).
+struct TstepF64 {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 8, kMaxIpl = 2, kOut = 2;
+  struct Params { double eta; };
+  using JRec = AccJerkF64::JRec;
+  struct I { double m, x, y, z, e2, vx, vy, vz; };
+  struct A64 { double sum, mx; };
+  __device__ static JRec pack(const Soa &s, int j, double sc) {
+    return AccJerkF64::pack(s, j, sc);
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[M][i],  s.f[RX][i], s.f[RY][i], s.f[RZ][i],
+            s.f[E2][i], s.f[VX][i], s.f[VY][i], s.f[VZ][i]};
+  }
+  __device__ static void zero(A64 &a) { a.sum = 0.0; a.mx = 0.0; }
+  __device__ static void values(const A64 &a, double *v) {
+    v[0] = a.sum;
+    v[1] = a.mx;
+  }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &prm) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double vx = q.vx - r.vx, vy = q.vy - r.vy, vz = q.vz - r.vz;
+    double m = q.m + r.m;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double rv = rx * vx + ry * vy + rz * vz;
+    double v2 = vx * vx + vy * vy + vz * vz;
+    const bool real_pair = r2 > 0.0;
+    double inv_r1 = rsq64(r2 + (q.e2 + r.e2));
+    inv_r1 = real_pair ? inv_r1 : 0.0;
+    double inv_r2 = inv_r1 * inv_r1;
+    double phi = m * inv_r1;
+    double w2 = inv_r2 * (v2 + 2.0 * phi);
+    double gam = inv_r2 * (w2 + 2.0 * inv_r2 * phi);
+    gam *= prm.eta * rsq64(w2);
+    w2 -= gam * rv;
+    // a masked pair may hold 0*inf = NaN here: select, do not multiply
+    w2 = real_pair ? w2 : 0.0;
+    a.sum += w2;
+    a.mx = fmax(w2, a.mx);
+  }
+  __device__ static void combine(double *v, const double *w) {
+    v[0] += w[0];
+    v[1] = fmax(w[1], v[1]);
+  }
+  __device__ static void finish(const double *v, Out out, int i,
+                                const Params &prm) {
+    out.f[0][i] = prm.eta / sqrt(1.0 + v[0]);
+    out.f[1][i] = prm.eta / sqrt(1.0 + v[1]);
+  }
+};
+
+}  // namespace tupan
+
+using namespace tupan;
+
+static hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <class P64, class P32c>
+static int dispatch(int kid, int mode, size_t ni, const double *const *ip,
+                    size_t nj, const double *const *jp, double *const *op,
+                    hipStream_t st) {
+  if (mode == TUPAN_MODE_FP64)
+    return run_engine<P64>(kid, mode, ni, ip, nj, jp, op, {}, st);
+  if (mode == TUPAN_MODE_FP32C)
+    return run_engine<P32c>(kid, mode, ni, ip, nj, jp, op, {}, st);
+  return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
+}
+
+template <class P64, class P32c>
+static int dispatch_host(int kid, int mode, int nfields, int nout, size_t ni,
+                         const double *const *ip, size_t nj,
+                         const double *const *jp, double *const *op) {
+  return host_call(ni, ip, nj, jp, nfields, op, nout,
+                   [&](const double *const *di, const double *const *dj,
+                       double *const *dout, hipStream_t st) {
+                     return dispatch<P64, P32c>(kid, mode, ni, di, nj, dj,
+                                                dout, st);
+                   });
+}
+
+extern "C" {
+
+int tupan_phi_dev(int mode, size_t ni, const double *const *ip, size_t nj,
+                  const double *const *jp, double *const *op, void *stream) {
+  return dispatch<PhiF64, PhiF32c>(TUPAN_K_PHI, mode, ni, ip, nj, jp, op,
+                                   as_stream(stream));
+}
+int tupan_phi_host(int mode, size_t ni, const double *const *ip, size_t nj,
+                   const double *const *jp, double *const *op) {
+  return dispatch_host<PhiF64, PhiF32c>(TUPAN_K_PHI, mode, 5, 1, ni, ip, nj,
+                                        jp, op);
+}
+
+int tupan_acc_dev(int mode, size_t ni, const double *const *ip, size_t nj,
+                  const double *const *jp, double *const *op, void *stream) {
+  return dispatch<AccF64, AccF32c>(TUPAN_K_ACC, mode, ni, ip, nj, jp, op,
+                                   as_stream(stream));
+}
+int tupan_acc_host(int mode, size_t ni, const double *const *ip, size_t nj,
+                   const double *const *jp, double *const *op) {
+  return dispatch_host<AccF64, AccF32c>(TUPAN_K_ACC, mode, 5, 3, ni, ip, nj,
+                                        jp, op);
+}
+
+int tupan_acc_jerk_dev(int mode, size_t ni, const double *const *ip,
+                       size_t nj, const double *const *jp, double *const *op,
+                       void *stream) {
+  return dispatch<AccJerkF64, AccJerkF32c>(TUPAN_K_ACC_JERK, mode, ni, ip, nj,
+                                           jp, op, as_stream(stream));
+}
+int tupan_acc_jerk_host(int mode, size_t ni, const double *const *ip,
+                        size_t nj, const double *const *jp,
+                        double *const *op) {
+  return dispatch_host<AccJerkF64, AccJerkF32c>(TUPAN_K_ACC_JERK, mode, 8, 6,
+                                                ni, ip, nj, jp, op);
+}
+
+int tupan_snap_crackle_dev(int mode, size_t ni, const double *const *ip,
+                           size_t nj, const double *const *jp,
+                           double *const *op, void *stream) {
+  return dispatch<SnapCrackleF64, SnapCrackleF32c>(
+      TUPAN_K_SNAP_CRACKLE, mode, ni, ip, nj, jp, op, as_stream(stream));
+}
+int tupan_snap_crackle_host(int mode, size_t ni, const double *const *ip,
+                            size_t nj, const double *const *jp,
+                            double *const *op) {
+  return dispatch_host<SnapCrackleF64, SnapCrackleF32c>(
+      TUPAN_K_SNAP_CRACKLE, mode, 14, 6, ni, ip, nj, jp, op);
+}
+
+int tupan_tstep_dev(size_t ni, const double *const *ip, size_t nj,
+                    const double *const *jp, double eta, double *const *op,
+                    void *stream) {
+  TstepF64::Params prm{eta};
+  return run_engine<TstepF64>(TUPAN_K_TSTEP, TUPAN_MODE_FP64, ni, ip, nj, jp,
+                              op, prm, as_stream(stream));
+}
+int tupan_tstep_host(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double eta, double *const *op) {
+  return host_call(ni, ip, nj, jp, 8, op, 2,
+                   [&](const double *const *di, const double *const *dj,
+                       double *const *dout, hipStream_t st) {
+                     TstepF64::Params prm{eta};
+                     return run_engine<TstepF64>(TUPAN_K_TSTEP,
+                                                 TUPAN_MODE_FP64, ni, di, nj,
+                                                 dj, dout, prm, st);
+                   });
+}
+
+}  // extern "C"

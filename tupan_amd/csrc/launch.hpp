@@ -1,0 +1,142 @@
+// launch.hpp -- host-side launch of the pair engine and the shared runtime
+// helpers (errors, launch-shape choice, host<->device staging).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstddef>
+#include <functional>
+
+#include "../../include/tupan_hip.h"
+#include "engine.hpp"
+
+namespace tupan {
+
+// ---- runtime.hip ----
+int set_error(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+struct LaunchShape {
+  int ipl;     // i-particles per lane
+  int splits;  // j-range splits
+};
+LaunchShape choose_shape(int kernel_id, int mode, size_t ni, size_t nj,
+                         int max_ipl);
+int device_alloc(void **p, size_t bytes, hipStream_t st);
+void device_free(void *p, hipStream_t st);
+// Optional live timing of the pair-engine launches (tupan_set_timing):
+// returns the event pair to record around the launch, or nulls when off.
+bool timing_begin(hipEvent_t *e0, hipEvent_t *e1);
+
+using DevFn = std::function<int(const double *const *, const double *const *,
+                                double *const *, hipStream_t)>;
+// Stages host arrays in HBM, runs fn on device pointers, copies the nout
+// outputs back and synchronises.  Fields shared by i and j (same host
+// pointer, same length) are copied once.
+int host_call(size_t ni, const double *const *ip, size_t nj,
+              const double *const *jp, int nfields, double *const *op,
+              int nout, const DevFn &fn);
+
+// ---- generic engine launch ----
+double near_field_factor();
+int sakura_budget();
+
+template <class Pol, int P>
+static void launch_p(dim3 grid, hipStream_t st, const Soa &is, const Soa &js,
+                     int ni, const typename Pol::JRec *jr, int nj, int jchunk,
+                     double *part, const Out &out,
+                     const typename Pol::Params &prm) {
+  hipLaunchKernelGGL((pair_engine<Pol, P>), grid, dim3(kBlock), 0, st, is,
+                     js, ni, jr, nj, jchunk, part, out, prm);
+}
+
+template <class Pol>
+int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
+               size_t nj, const double *const *jp, double *const *op,
+               const typename Pol::Params &prm, hipStream_t st) {
+  if (ni == 0) return TUPAN_OK;
+  if (!ip || !jp || !op) return set_error(TUPAN_ERR_ARG, "NULL field array");
+  if (ni > (size_t)INT_MAX / 8 || nj > (size_t)INT_MAX / 8)
+    return set_error(TUPAN_ERR_ARG, "ni/nj too large (%zu, %zu)", ni, nj);
+  Soa is{}, js{};
+  for (int k = 0; k < Pol::kFields; ++k) {
+    if (!ip[k] || (nj && !jp[k]))
+      return set_error(TUPAN_ERR_ARG, "NULL input field %d", k);
+    is.f[k] = ip[k];
+    js.f[k] = jp[k];
+  }
+  Out out{};
+  for (int k = 0; k < Pol::kOut; ++k) {
+    if (!op[k]) return set_error(TUPAN_ERR_ARG, "NULL output field %d", k);
+    out.f[k] = op[k];
+  }
+  const int n_i = (int)ni, n_j = (int)nj;
+  LaunchShape sh = choose_shape(kernel_id, mode, ni, nj, Pol::kMaxIpl);
+  const int P = sh.ipl;
+  int S = sh.splits;
+  int jchunk = n_j;
+  if (S > 1) {
+    jchunk = (n_j + S - 1) / S;
+    jchunk = (jchunk + kTile - 1) / kTile * kTile;
+    S = (n_j + jchunk - 1) / jchunk;
+  }
+  if (S < 1) S = 1;
+
+  typename Pol::JRec *jr = nullptr;
+  double *part = nullptr;
+  double *scale = nullptr;
+  int rc;
+  if (n_j > 0) {
+    rc = device_alloc((void **)&jr,
+                      sizeof(typename Pol::JRec) * (size_t)n_j + 256, st);
+    if (rc) return rc;
+    // the near-field factor lives in the 256 bytes after the records
+    scale = reinterpret_cast<double *>(
+        reinterpret_cast<char *>(jr) +
+        ((sizeof(typename Pol::JRec) * (size_t)n_j + 127) / 128) * 128);
+  }
+  if (S > 1) {
+    rc = device_alloc((void **)&part,
+                      sizeof(double) * (size_t)S * Pol::kOut * (size_t)n_i,
+                      st);
+    if (rc) {
+      device_free(jr, st);
+      return rc;
+    }
+  }
+  if (n_j > 0) {
+    const bool near = Pol::kTwoLevel && near_field_factor() > 0.0;
+    if (near)
+      hipLaunchKernelGGL(near_scale_kernel, dim3(1), dim3(1024), 0, st,
+                         js.f[M], n_j, near_field_factor(), scale);
+    hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
+                       dim3(256), 0, st, js, n_j, jr,
+                       near ? (const double *)scale : nullptr);
+  }
+  const int waves = (n_i + 64 * P - 1) / (64 * P);
+  dim3 grid((waves + kWaves - 1) / kWaves, S);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const bool timed = timing_begin(&ev0, &ev1);
+  if (timed) (void)hipEventRecord(ev0, st);
+  switch (P) {
+    case 4:
+      launch_p<Pol, (Pol::kMaxIpl >= 4 ? 4 : 1)>(grid, st, is, js, n_i, jr, n_j,
+                                                  jchunk, part, out, prm);
+      break;
+    case 2:
+      launch_p<Pol, (Pol::kMaxIpl >= 2 ? 2 : 1)>(grid, st, is, js, n_i, jr, n_j,
+                                                  jchunk, part, out, prm);
+      break;
+    default:
+      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, part, out, prm);
+  }
+  if (timed) (void)hipEventRecord(ev1, st);
+  if (S > 1)
+    hipLaunchKernelGGL((finish_kernel<Pol>), dim3((n_i + 255) / 256),
+                       dim3(256), 0, st, part, S, n_i, out, prm);
+  hipError_t e = hipGetLastError();
+  device_free(part, st);
+  device_free(jr, st);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return TUPAN_OK;
+}
+
+}  // namespace tupan

@@ -1,0 +1,229 @@
+/*
+ * libtupan_abi.c -- drop-in replacement for the reference's kernel
+ * libraries.  Built twice (tupan_amd/build.py):
+ *
+ *   libtupan_f64.so  (-DTUPAN_ABI_F64): REAL = double, INT = long,
+ *                    UINT = unsigned long
+ *   libtupan_f32.so                   : REAL = float, INT = int,
+ *                    UINT = unsigned int
+ *
+ * exactly as the reference builds its FP64 and FP32 libraries with the same
+ * symbol names (tupan/lib/cffi_backend.py:31-87, tupan/lib/src/c_common.h:
+ * 7-43).  Each exported function has the signature of its declaration in
+ * tupan/lib/src/libtupan.h and forwards to the synchronous host-pointer
+ * entry points of libtupan_hip.so with the library's default mode
+ * (tupan_set_default_mode).  FP32 arrays are widened to FP64 for the device
+ * and the results narrowed back.
+ *
+ * The reference functions return void and report nothing.  On a device or
+ * argument failure these shims write NaN to every output, print the error
+ * once to stderr and record it for tupan_abi_last_status().
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "libtupan_abi.h"
+#include "tupan_hip.h"
+
+#define EXPORT __attribute__((visibility("default")))
+
+static int g_status = 0;
+
+EXPORT int tupan_abi_last_status(void) { return g_status; }
+
+static void report(int rc, const char *who, size_t ni, REAL **outs, int nout)
+{
+    g_status = rc;
+    if (rc == TUPAN_OK) return;
+    static int printed = 0;
+    if (!printed) {
+        fprintf(stderr, "%s: %s\n", who, tupan_last_error());
+        printed = 1;
+    }
+    if (rc == TUPAN_ERR_NOT_CONVERGED) return; /* results are valid */
+    for (int k = 0; k < nout; ++k)
+        for (size_t i = 0; i < ni; ++i) outs[k][i] = (REAL)NAN;
+}
+
+#ifdef TUPAN_ABI_F64
+/* FP64: pass the caller's arrays straight through. */
+#define WIDEN(dst, src, n) ((dst) = (const double *)(src), 0)
+#define UNWIDEN(dst, src, n) ((void)0)
+#define ALLOC_OUT(dst, src, n) ((dst) = (double *)(src), 0)
+#define FREE_W(p) ((void)0)
+#define FREE_O(p) ((void)0)
+#else
+static int widen(const double **dst, const float *src, size_t n)
+{
+    double *d = (double *)malloc((n ? n : 1) * sizeof(double));
+    if (!d) return 1;
+    for (size_t i = 0; i < n; ++i) d[i] = (double)src[i];
+    *dst = d;
+    return 0;
+}
+static int alloc_out(double **dst, size_t n)
+{
+    *dst = (double *)malloc((n ? n : 1) * sizeof(double));
+    return *dst == NULL;
+}
+#define WIDEN(dst, src, n) widen(&(dst), (src), (n))
+#define ALLOC_OUT(dst, src, n) alloc_out(&(dst), (n))
+#define UNWIDEN(dst, src, n) \
+    do { for (size_t q_ = 0; q_ < (n); ++q_) (dst)[q_] = (float)(src)[q_]; } while (0)
+#define FREE_W(p) free((void *)(p))
+#define FREE_O(p) free((void *)(p))
+#endif
+
+typedef int (*mode_fn)(int, size_t, const double *const *, size_t,
+                       const double *const *, double *const *);
+
+/* Generic body: nf input fields per side, nout outputs. */
+static void forward(const char *who, int kind, mode_fn fn, size_t ni,
+                    const REAL *const *ih, size_t nj, const REAL *const *jh,
+                    int nf, REAL **oh, int nout, double eta, unsigned order,
+                    const double *inv, double dt, int flag)
+{
+    const double *iw[14] = {0}, *jw[14] = {0};
+    double *ow[6] = {0};
+    int rc = TUPAN_OK, bad = 0;
+    if (ni == 0) return;
+    for (int k = 0; k < nf && !bad; ++k) bad |= WIDEN(iw[k], ih[k], ni);
+    for (int k = 0; k < nf && !bad; ++k) {
+#ifdef TUPAN_ABI_F64
+        bad |= WIDEN(jw[k], jh[k], nj);
+#else
+        if (jh[k] == ih[k] && nj == ni) jw[k] = iw[k];
+        else bad |= WIDEN(jw[k], jh[k], nj);
+#endif
+    }
+    for (int k = 0; k < nout && !bad; ++k) bad |= ALLOC_OUT(ow[k], oh[k], ni);
+    if (bad) {
+        rc = TUPAN_ERR_ARG;
+    } else {
+        int mode = tupan_get_default_mode();
+        switch (kind) {
+        case 0: rc = fn(mode, ni, iw, nj, jw, ow); break;
+        case 1: rc = tupan_tstep_host(ni, iw, nj, jw, eta, ow); break;
+        case 2: rc = tupan_pnacc_host(ni, iw, nj, jw, order, inv, ow); break;
+        case 3: rc = tupan_sakura_host(ni, iw, nj, jw, dt, flag, ow); break;
+        }
+    }
+    for (int k = 0; k < nout; ++k) {
+        if (ow[k]) UNWIDEN(oh[k], ow[k], ni);
+        FREE_O(ow[k]);
+    }
+#ifndef TUPAN_ABI_F64
+    for (int k = 0; k < nf; ++k) {
+        if (jw[k] && jw[k] != iw[k]) FREE_W(jw[k]);
+        if (iw[k]) FREE_W(iw[k]);
+    }
+#endif
+    report(rc, who, ni, oh, nout);
+}
+
+#define R const REAL *restrict
+
+/* libtupan.h:2-15 */
+EXPORT void phi_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                       const UINT nj, R jm, R jrx, R jry, R jrz, R je2,
+                       REAL *restrict iphi)
+{
+    const REAL *i5[5] = {im, irx, iry, irz, ie2};
+    const REAL *j5[5] = {jm, jrx, jry, jrz, je2};
+    REAL *o[1] = {iphi};
+    forward("phi_kernel", 0, tupan_phi_host, ni, i5, nj, j5, 5, o, 1, 0, 0,
+            NULL, 0, 0);
+}
+
+/* libtupan.h:17-32 */
+EXPORT void acc_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                       const UINT nj, R jm, R jrx, R jry, R jrz, R je2,
+                       REAL *restrict iax, REAL *restrict iay,
+                       REAL *restrict iaz)
+{
+    const REAL *i5[5] = {im, irx, iry, irz, ie2};
+    const REAL *j5[5] = {jm, jrx, jry, jrz, je2};
+    REAL *o[3] = {iax, iay, iaz};
+    forward("acc_kernel", 0, tupan_acc_host, ni, i5, nj, j5, 5, o, 3, 0, 0,
+            NULL, 0, 0);
+}
+
+/* libtupan.h:34-58 */
+EXPORT void acc_jerk_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                            R ivx, R ivy, R ivz, const UINT nj, R jm, R jrx,
+                            R jry, R jrz, R je2, R jvx, R jvy, R jvz,
+                            REAL *restrict iax, REAL *restrict iay,
+                            REAL *restrict iaz, REAL *restrict ijx,
+                            REAL *restrict ijy, REAL *restrict ijz)
+{
+    const REAL *i8[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const REAL *j8[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    REAL *o[6] = {iax, iay, iaz, ijx, ijy, ijz};
+    forward("acc_jerk_kernel", 0, tupan_acc_jerk_host, ni, i8, nj, j8, 8, o,
+            6, 0, 0, NULL, 0, 0);
+}
+
+/* libtupan.h:60-96 */
+EXPORT void snap_crackle_kernel(
+    const UINT ni, R im, R irx, R iry, R irz, R ie2, R ivx, R ivy, R ivz,
+    R iax, R iay, R iaz, R ijx, R ijy, R ijz, const UINT nj, R jm, R jrx,
+    R jry, R jrz, R je2, R jvx, R jvy, R jvz, R jax, R jay, R jaz, R jjx,
+    R jjy, R jjz, REAL *restrict isx, REAL *restrict isy, REAL *restrict isz,
+    REAL *restrict icx, REAL *restrict icy, REAL *restrict icz)
+{
+    const REAL *i14[14] = {im, irx, iry, irz, ie2, ivx, ivy, ivz,
+                           iax, iay, iaz, ijx, ijy, ijz};
+    const REAL *j14[14] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz,
+                           jax, jay, jaz, jjx, jjy, jjz};
+    REAL *o[6] = {isx, isy, isz, icx, icy, icz};
+    forward("snap_crackle_kernel", 0, tupan_snap_crackle_host, ni, i14, nj,
+            j14, 14, o, 6, 0, 0, NULL, 0, 0);
+}
+
+/* libtupan.h:98-119 */
+EXPORT void tstep_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                         R ivx, R ivy, R ivz, const UINT nj, R jm, R jrx,
+                         R jry, R jrz, R je2, R jvx, R jvy, R jvz,
+                         const REAL eta, REAL *restrict idt_a,
+                         REAL *restrict idt_b)
+{
+    const REAL *i8[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const REAL *j8[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    REAL *o[2] = {idt_a, idt_b};
+    forward("tstep_kernel", 1, NULL, ni, i8, nj, j8, 8, o, 2, (double)eta, 0,
+            NULL, 0, 0);
+}
+
+/* libtupan.h:121-150 */
+EXPORT void pnacc_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                         R ivx, R ivy, R ivz, const UINT nj, R jm, R jrx,
+                         R jry, R jrz, R je2, R jvx, R jvy, R jvz, UINT order,
+                         const REAL inv1, const REAL inv2, const REAL inv3,
+                         const REAL inv4, const REAL inv5, const REAL inv6,
+                         const REAL inv7, REAL *restrict ipnax,
+                         REAL *restrict ipnay, REAL *restrict ipnaz)
+{
+    const REAL *i8[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const REAL *j8[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    REAL *o[3] = {ipnax, ipnay, ipnaz};
+    double inv[7] = {inv1, inv2, inv3, inv4, inv5, inv6, inv7};
+    forward("pnacc_kernel", 2, NULL, ni, i8, nj, j8, 8, o, 3, 0,
+            (unsigned)order, inv, 0, 0);
+}
+
+/* libtupan.h:203-229 */
+EXPORT void sakura_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                          R ivx, R ivy, R ivz, const UINT nj, R jm, R jrx,
+                          R jry, R jrz, R je2, R jvx, R jvy, R jvz,
+                          const REAL dt, const INT flag, REAL *restrict idrx,
+                          REAL *restrict idry, REAL *restrict idrz,
+                          REAL *restrict idvx, REAL *restrict idvy,
+                          REAL *restrict idvz)
+{
+    const REAL *i8[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const REAL *j8[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    REAL *o[6] = {idrx, idry, idrz, idvx, idvy, idvz};
+    forward("sakura_kernel", 3, NULL, ni, i8, nj, j8, 8, o, 6, 0, 0, NULL,
+            (double)dt, (int)flag);
+}

@@ -1,0 +1,269 @@
+// runtime.hip -- error reporting, launch-shape selection, device memory and
+// host<->HBM staging shared by every kernel of the library.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "launch.hpp"
+
+namespace tupan {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return set_error(TUPAN_ERR_HIP, "%s: %s (%d)", what, hipGetErrorString(e),
+                   (int)e);
+}
+
+// ------------------------------------------------------ launch shapes
+static std::mutex g_mu;
+static LaunchShape g_override[8][2];  // [kernel][mode], zero = automatic
+static int g_default_mode = TUPAN_MODE_FP64;
+static double g_near_c = 2.0;          // near-field radius, see engine.hpp
+static int g_sakura_budget = 1 << 20;  // Kepler solves per Sakura pair
+
+double near_field_factor() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_near_c;
+}
+
+int sakura_budget() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_sakura_budget;
+}
+
+static int simd_count() {
+  static int cached = 0;
+  if (cached) return cached;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            dev) == hipSuccess &&
+      cus > 0)
+    cached = 4 * cus;  // 4 SIMDs per CU on CDNA
+  else
+    cached = 1024;
+  return cached;
+}
+
+LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
+                         int max_ipl) {
+  LaunchShape o{0, 0};
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (kid >= 0 && kid < 8 && (mode == 0 || mode == 1)) o = g_override[kid][mode];
+  }
+  // Aim for 4 resident waves per SIMD; never split j below 256 per wave.
+  const long target = 4L * simd_count();
+  const long max_split = nj >= 512 ? (long)(nj / 256) : 1;
+  int P = o.ipl > 0 ? o.ipl : max_ipl;
+  if (P > max_ipl) P = max_ipl;
+  if (P != 1 && P != 2 && P != 4) P = 1;
+  auto waves_i = [&](int p) { return (long)((ni + 64 * p - 1) / (64 * p)); };
+  if (o.ipl <= 0)
+    while (P > 1 && waves_i(P) * max_split < target) P /= 2;
+  long S = o.splits > 0 ? o.splits : (target + waves_i(P) - 1) / waves_i(P);
+  if (S > max_split && o.splits <= 0) S = max_split;
+  if (S < 1) S = 1;
+  if (nj > 0 && S > (long)nj) S = (long)nj;
+  return LaunchShape{P, (int)S};
+}
+
+// ------------------------------------------------------ device memory
+// Stream-ordered allocation from the device's default memory pool; the
+// pool keeps freed blocks (release threshold raised once per device), so
+// steady-state calls do not reach the driver.
+int device_alloc(void **p, size_t bytes, hipStream_t st) {
+  static bool pool_set[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev >= 0 && dev < 64 && !pool_set[dev]) {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t thr = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    pool_set[dev] = true;
+  }
+  if (bytes == 0) bytes = 8;
+  e = hipMallocAsync(p, bytes, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+  return TUPAN_OK;
+}
+
+void device_free(void *p, hipStream_t st) {
+  if (p) (void)hipFreeAsync(p, st);
+}
+
+// ------------------------------------------------------ live timing
+static bool g_timing = false;
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
+
+bool timing_begin(hipEvent_t *e0, hipEvent_t *e1) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_timing) return false;
+  if (hipEventCreate(e0) != hipSuccess) return false;
+  if (hipEventCreate(e1) != hipSuccess) {
+    (void)hipEventDestroy(*e0);
+    return false;
+  }
+  g_events.emplace_back(*e0, *e1);
+  return true;
+}
+
+// ------------------------------------------------------ host staging
+int host_call(size_t ni, const double *const *ip, size_t nj,
+              const double *const *jp, int nf, double *const *op, int nout,
+              const DevFn &fn) {
+  if (ni == 0) return TUPAN_OK;
+  if (!ip || !jp || !op) return set_error(TUPAN_ERR_ARG, "NULL field array");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1)
+    return set_error(TUPAN_ERR_HIP, "no HIP device available (%s)",
+                     hipGetErrorString(e));
+  hipStream_t st = nullptr;
+  std::vector<double *> di(nf, nullptr), dj(nf, nullptr), dout(nout, nullptr);
+  std::vector<void *> owned;
+  int rc = TUPAN_OK;
+  auto stage = [&](const double *h, size_t n, double **d) -> int {
+    int r = device_alloc((void **)d, n * sizeof(double), st);
+    if (r) return r;
+    owned.push_back(*d);
+    hipError_t ce =
+        hipMemcpyAsync(*d, h, n * sizeof(double), hipMemcpyHostToDevice, st);
+    return ce == hipSuccess ? TUPAN_OK : hip_fail(ce, "H2D copy");
+  };
+  for (int k = 0; k < nf && !rc; ++k) {
+    if (!ip[k]) { rc = set_error(TUPAN_ERR_ARG, "NULL i field %d", k); break; }
+    rc = stage(ip[k], ni, &di[k]);
+  }
+  for (int k = 0; k < nf && !rc && nj; ++k) {
+    if (!jp[k]) { rc = set_error(TUPAN_ERR_ARG, "NULL j field %d", k); break; }
+    if (jp[k] == ip[k] && nj == ni)
+      dj[k] = di[k];
+    else
+      rc = stage(jp[k], nj, &dj[k]);
+  }
+  for (int k = 0; k < nout && !rc; ++k) {
+    if (!op[k]) { rc = set_error(TUPAN_ERR_ARG, "NULL output %d", k); break; }
+    rc = device_alloc((void **)&dout[k], ni * sizeof(double), st);
+    if (!rc) owned.push_back(dout[k]);
+  }
+  if (!rc) rc = fn(di.data(), dj.data(), dout.data(), st);
+  int rc_keep = rc;
+  if (rc == TUPAN_OK || rc == TUPAN_ERR_NOT_CONVERGED) {
+    for (int k = 0; k < nout; ++k) {
+      hipError_t ce = hipMemcpyAsync(op[k], dout[k], ni * sizeof(double),
+                                     hipMemcpyDeviceToHost, st);
+      if (ce != hipSuccess) { rc_keep = hip_fail(ce, "D2H copy"); break; }
+    }
+  }
+  for (void *p : owned) device_free(p, st);
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess && rc_keep == TUPAN_OK)
+    rc_keep = hip_fail(e, "hipStreamSynchronize");
+  return rc_keep;
+}
+
+}  // namespace tupan
+
+using namespace tupan;
+
+extern "C" {
+
+const char *tupan_version(void) { return "tupan_hip 0.1.0 (gfx950)"; }
+
+const char *tupan_last_error(void) { return g_err; }
+
+int tupan_device_count(int *count) {
+  if (!count) return set_error(TUPAN_ERR_ARG, "NULL count");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = (e == hipSuccess) ? n : 0;
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+  return TUPAN_OK;
+}
+
+int tupan_set_default_mode(int mode) {
+  if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
+    return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_default_mode = mode;
+  return TUPAN_OK;
+}
+
+int tupan_get_default_mode(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_default_mode;
+}
+
+int tupan_set_near_field(double c) {
+  if (!(c >= 0.0) || c > 1e6)
+    return set_error(TUPAN_ERR_ARG, "near-field factor must be in [0, 1e6]");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_near_c = c;
+  return TUPAN_OK;
+}
+
+int tupan_set_sakura_budget(int solves) {
+  if (solves < 1) return set_error(TUPAN_ERR_ARG, "budget must be >= 1");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_sakura_budget = solves;
+  return TUPAN_OK;
+}
+
+int tupan_set_timing(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_timing = on != 0;
+  return TUPAN_OK;
+}
+
+int tupan_take_kernel_times(double *total_ms, int *count) {
+  if (!total_ms || !count) return set_error(TUPAN_ERR_ARG, "NULL output");
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ev.swap(g_events);
+  }
+  double sum = 0.0;
+  int rc = TUPAN_OK;
+  for (auto &p : ev) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(p.second);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.first, p.second);
+    if (e != hipSuccess && rc == TUPAN_OK) rc = hip_fail(e, "event timing");
+    sum += ms;
+    (void)hipEventDestroy(p.first);
+    (void)hipEventDestroy(p.second);
+  }
+  *total_ms = sum;
+  *count = (int)ev.size();
+  return rc;
+}
+
+int tupan_set_launch(int kernel_id, int mode, int ipl, int splits) {
+  if (kernel_id < 0 || kernel_id >= 8)
+    return set_error(TUPAN_ERR_ARG, "bad kernel id %d", kernel_id);
+  if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
+    return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
+  if (ipl != 0 && ipl != 1 && ipl != 2 && ipl != 4)
+    return set_error(TUPAN_ERR_ARG, "ipl must be 0, 1, 2 or 4");
+  if (splits < 0) return set_error(TUPAN_ERR_ARG, "splits must be >= 0");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_override[kernel_id][mode] = LaunchShape{ipl, splits};
+  return TUPAN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// sakura.hip -- Sakura pairwise two-body drift (FP64) on the pair engine.
+//
+// For every pair the relative orbit (r0, v0) = (x_i - x_j, v_i - v_j) with
+// m = m_i + m_j is evolved by pair_evolve (twobody.hpp) and the i-particle
+// receives dr_i += m_j/m (r1 - r0), dv_i += m_j/m (v1 - v0)
+// (tupan/lib/src/sakura_kernel_common.h:194-242).  Lanes whose pairs need
+// Kepler retries run while the others idle: the cost is data-dependent by
+// nature.  A pair that exhausts its solve budget sets bit 0 of the caller's
+// status word.
+#include "launch.hpp"
+#include "twobody.hpp"
+
+namespace tupan {
+
+struct SakuraParams {
+  double dt;
+  int flag;
+  int budget;
+  int *status;
+};
+
+struct SakuraF64 : SumOut<6> {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 8, kMaxIpl = 1;
+  using Params = SakuraParams;
+  struct JRec { double m, x, y, z, e2, vx, vy, vz; };
+  using I = JRec;
+  using A64 = F64Acc<6>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
+            s.f[E2][j], s.f[VX][j], s.f[VY][j], s.f[VZ][j]};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return pack(s, i, 0.0);
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &prm) {
+    double st[6] = {q.x - r.x,   q.y - r.y,   q.z - r.z,
+                    q.vx - r.vx, q.vy - r.vy, q.vz - r.vz};
+    const double r0[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
+    double m = q.m + r.m;
+    double e2 = q.e2 + r.e2;
+    if (!pair_evolve(prm.dt, prm.flag, m, e2, st, prm.budget) && prm.status)
+      atomicOr(prm.status, 1);
+    double w = r.m / m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.v[k] += w * (st[k] - r0[k]);
+  }
+};
+
+}  // namespace tupan
+
+using namespace tupan;
+
+extern "C" {
+
+int tupan_sakura_dev(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, int flag,
+                     double *const *op, int *d_status, void *stream) {
+  SakuraParams prm{dt, flag, sakura_budget(), d_status};
+  return run_engine<SakuraF64>(TUPAN_K_SAKURA, TUPAN_MODE_FP64, ni, ip, nj,
+                               jp, op, prm,
+                               reinterpret_cast<hipStream_t>(stream));
+}
+
+int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double dt, int flag,
+                      double *const *op) {
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1)
+    return set_error(TUPAN_ERR_HIP, "no HIP device available");
+  int *status = nullptr;
+  int rc = device_alloc((void **)&status, sizeof(int), nullptr);
+  if (rc) return rc;
+  hipError_t e = hipMemsetAsync(status, 0, sizeof(int), nullptr);
+  if (e != hipSuccess) {
+    device_free(status, nullptr);
+    return hip_fail(e, "hipMemsetAsync");
+  }
+  rc = host_call(ni, ip, nj, jp, 8, op, 6,
+                 [&](const double *const *di, const double *const *dj,
+                     double *const *dout, hipStream_t st) {
+                   SakuraParams prm{dt, flag, sakura_budget(), status};
+                   return run_engine<SakuraF64>(TUPAN_K_SAKURA,
+                                                TUPAN_MODE_FP64, ni, di, nj,
+                                                dj, dout, prm, st);
+                 });
+  int hstat = 0;
+  e = hipMemcpy(&hstat, status, sizeof(int), hipMemcpyDeviceToHost);
+  device_free(status, nullptr);
+  (void)hipStreamSynchronize(nullptr);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "status copy");
+  if (hstat)
+    return set_error(TUPAN_ERR_NOT_CONVERGED,
+                     "a Kepler solve exhausted its budget of %d solves",
+                     sakura_budget());
+  return TUPAN_OK;
+}
+
+}  // extern "C"

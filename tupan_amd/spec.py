@@ -1,0 +1,44 @@
+"""Kernel table: field lists, outputs and flop conventions.
+
+Field order is the reference's argument order (tupan/lib/src/libtupan.h);
+output attribute names are the ones tupan/lib/extensions.py attaches to the
+i-particles (Phi :132, Acc :198, AccJerk :279-280, SnapCrackle :337-338,
+Tstep :382, PNAcc :434, Sakura :501-502).
+
+``flops`` is the reference's own per-interaction count from the comments
+of each *_kernel_common.h (phi :32, acc :38, acc_jerk :56, snap_crackle :95,
+tstep :60); PNAcc and Sakura have none ("40+???", "36 + ?").
+"""
+from collections import namedtuple
+
+F5 = ("mass", "rx", "ry", "rz", "eps2")
+F8 = F5 + ("vx", "vy", "vz")
+F14 = F8 + ("ax", "ay", "az", "jx", "jy", "jz")
+
+Kernel = namedtuple("Kernel", "name kid fields outs has_mode flops")
+
+KERNELS = {
+    "phi": Kernel("phi", 0, F5, ("phi",), True, 14),
+    "acc": Kernel("acc", 1, F5, ("ax", "ay", "az"), True, 20),
+    "acc_jerk": Kernel("acc_jerk", 2, F8,
+                       ("ax", "ay", "az", "jx", "jy", "jz"), True, 42),
+    "snap_crackle": Kernel("snap_crackle", 3, F14,
+                           ("sx", "sy", "sz", "cx", "cy", "cz"), True, 114),
+    "tstep": Kernel("tstep", 4, F8, ("tstep", "tstepij"), False, 42),
+    "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), False, None),
+    "sakura": Kernel("sakura", 6, F8,
+                     ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
+}
+
+KERNEL_IDS = {k: v.kid for k, v in KERNELS.items()}
+
+# reference C symbol -> kernel key
+SYMBOLS = {
+    "phi_kernel": "phi",
+    "acc_kernel": "acc",
+    "acc_jerk_kernel": "acc_jerk",
+    "snap_crackle_kernel": "snap_crackle",
+    "tstep_kernel": "tstep",
+    "pnacc_kernel": "pnacc",
+    "sakura_kernel": "sakura",
+}
