@@ -49,7 +49,28 @@ def parse():
                    help="target CPU work for the cpu_baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--pmc-target", action="store_true",
+                   help="one evaluation per mode plus a 256 MiB device copy "
+                        "(the byte-counter calibration kernel), for "
+                        "rocprofv3 --pmc runs; prints no JSON")
     return p.parse_args()
+
+
+PMC_PROFILE = os.path.join(ROOT, "profiles", "acc_jerk_n65536_pmc.md")
+
+
+def measured_traffic(mode):
+    """HBM bytes per pair-engine launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel factor);
+    None when that summary is absent."""
+    key = "traffic_bytes_%s:" % mode
+    try:
+        for line in open(PMC_PROFILE):
+            if line.strip().startswith(key):
+                return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def make_workload(n, seed=1):
@@ -135,6 +156,16 @@ def main():
     ps = make_workload(n)
     d = device.to_device(ps, spec.F8)
     outs = tuple(torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(6))
+    if args.pmc_target:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # 256 MiB read + 256 MiB written
+        for mode in ("fp32c", "fp64"):
+            device.compute("acc_jerk", d, d, mode=mode, out=outs)
+        torch.cuda.synchronize()
+        print("[bench] pmc target done", file=sys.stderr, flush=True)
+        return
 
     def step():
         device.compute("acc_jerk", d, d, mode=args.mode, out=outs)
@@ -169,7 +200,8 @@ def main():
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None, "kernel_ms": k_ms, "kernel_launches": k_n,
+            "traffic": measured_traffic(args.mode) if n == 65536 else None,
+            "kernel_ms": k_ms, "kernel_launches": k_n,
             "flops_per_interaction": FLOPS_PER_PAIR,
         },
     }

@@ -50,6 +50,15 @@ class Golden(object):
             k += 1
         return out
 
+    def abs_sums(self, case):
+        """S_i = sum_j ||f_ij|| per output group, or None (Tstep, FP32)."""
+        out = []
+        g = 0
+        while "abs:%d:%d" % (case["id"], g) in self.z.files:
+            out.append(self.z["abs:%d:%d" % (case["id"], g)])
+            g += 1
+        return out or None
+
     @staticmethod
     def args(case):
         """Scalar args in the form tupan_amd.kernels.compute takes."""
@@ -85,6 +94,17 @@ def rel_norm_err(got, ref, groups):
         scale = np.maximum(r, 1e-300 * max(r.max(), 1e-300))
         if len(r):
             worst = max(worst, float(np.max(d / scale)))
+    return worst
+
+
+def bound_ratio(got, ref, S, groups, rel, absol):
+    """max_i ||got_i - ref_i|| / (rel ||ref_i|| + absol S_i) over groups."""
+    worst = 0.0
+    for g, grp in enumerate(groups):
+        d = np.sqrt(sum((np.asarray(got[k], np.float64) - ref[k])**2 for k in grp))
+        r = np.sqrt(sum(np.asarray(ref[k], np.float64)**2 for k in grp))
+        if len(d):
+            worst = max(worst, float(np.max(d / (rel * r + absol * S[g] + 1e-300))))
     return worst
 
 

@@ -11,7 +11,9 @@ shapes (tupan/tests/test_extensions.py:39-127: random sets, prefix slices
 ps[:k] on either side, eta = 1/64, PN order 7 at clight = 128) and add the
 edge cases that test leaves open (coincident particles, softened sets, PN
 orders 0..7, every Sakura flag with close, wide, bound and unbound pairs,
-the Plummer N=256 seed-0 model).
+the Plummer N=256 seed-0 model).  For every FP64 sum kernel the file also
+holds, per particle and output vector, S_i = sum_j ||f_ij||, the magnitude
+scale of its pair terms (the condition of the sum), from the same library.
 """
 import json
 import os
@@ -81,6 +83,27 @@ def plummer_set(ref):
     for f, v in zip(("ax", "ay", "az", "jx", "jy", "jz"), aj):
         d[f] = v
     return d
+
+
+# output vector groups per kernel (Tstep's outputs are not plain sums)
+GROUPS = {"phi": [(0,)], "acc": [(0, 1, 2)],
+          "acc_jerk": [(0, 1, 2), (3, 4, 5)],
+          "snap_crackle": [(0, 1, 2), (3, 4, 5)],
+          "pnacc": [(0, 1, 2)], "sakura": [(0, 1, 2), (3, 4, 5)]}
+
+
+def abs_sums(lib, name, ips, jps, args):
+    """Per i and output group, sum_j ||f_ij||: the magnitude scale of the
+    pair terms, i.e. the condition of the sum (one j at a time)."""
+    groups = GROUPS[SHORT[name]]
+    ni = len(ips["mass"])
+    S = [np.zeros(ni) for _ in groups]
+    for j in range(len(jps["mass"])):
+        one = {f: v[j:j + 1] for f, v in jps.items()}
+        c = lib.call(name, ips, one, *args)
+        for g, grp in enumerate(groups):
+            S[g] += np.sqrt(sum(np.asarray(c[k], np.float64)**2 for k in grp))
+    return S
 
 
 def sl(d, k, n):
@@ -157,6 +180,10 @@ def main():
             ips = {f: v.astype(np.float32) for f, v in ips.items()}
             jps = {f: v.astype(np.float32) for f, v in jps.items()}
         res = lib.call(cs["kernel"], ips, jps, *cs["args"])
+        if cs["prec"] == "float64" and SHORT[cs["kernel"]] in GROUPS:
+            for g, v in enumerate(abs_sums(lib, cs["kernel"], ips, jps,
+                                           cs["args"])):
+                out["abs:%d:%d" % (cid, g)] = v
         print(cid, cs["kernel"], cs["set"], cs["prec"], flush=True)
         cs["id"] = cid
         cs["kernel"] = SHORT[cs["kernel"]]

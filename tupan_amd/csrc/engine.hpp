@@ -126,22 +126,24 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
         Pol::zero(a[p]);
         near[p] = 0u;
       }
+      // the record of j+1 is fetched (s_load) while j is computed
+      typename Pol::JRec r = jr[j0];
       if (nk == kTile) {
 #pragma unroll 2
         for (int k = 0; k < kTile; ++k) {
-          const typename Pol::JRec r = jr[j0 + k];
+          const typename Pol::JRec rn = jr[j0 + min(k + 1, kTile - 1)];
 #pragma unroll
-          for (int p = 0; p < P; ++p) {
+          for (int p = 0; p < P; ++p)
             near[p] |= near_bit(Pol::pair32(ist[p], r, a[p])) << k;
-          }
+          r = rn;
         }
       } else {
         for (int k = 0; k < nk; ++k) {
-          const typename Pol::JRec r = jr[j0 + k];
+          const typename Pol::JRec rn = jr[j0 + min(k + 1, nk - 1)];
 #pragma unroll
-          for (int p = 0; p < P; ++p) {
+          for (int p = 0; p < P; ++p)
             near[p] |= near_bit(Pol::pair32(ist[p], r, a[p])) << k;
-          }
+          r = rn;
         }
       }
 #pragma unroll
@@ -155,12 +157,14 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
         }
       }
     }
-  } else {
+  } else if (jb < je) {
+    typename Pol::JRec r = jr[jb];
 #pragma unroll 2
     for (int j = jb; j < je; ++j) {
-      const typename Pol::JRec r = jr[j];
+      const typename Pol::JRec rn = jr[min(j + 1, je - 1)];
 #pragma unroll
       for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
+      r = rn;
     }
   }
 

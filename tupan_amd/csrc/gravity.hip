@@ -81,7 +81,7 @@ __device__ __forceinline__ float near2_of(double m, double sc) {
 // ================================================================ Phi
 struct PhiF64 : SumOut<1> {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 5, kMaxIpl = 4;
+  static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using Params = NoParams;
   struct JRec { double m, x, y, z, e2; };
   struct I { double x, y, z, e2; };
@@ -118,7 +118,7 @@ __device__ __forceinline__ Pos32cI load_pos32c(const Soa &s, int i) {
 }
 
 struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
-  static constexpr int kFields = 5, kMaxIpl = 4;
+  static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using JRec = Pos32cJ;
   using I = Pos32cI;
   __device__ static JRec pack(const Soa &s, int j, double sc) {
@@ -138,7 +138,7 @@ struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
 // ================================================================ Acc
 struct AccF64 : SumOut<3> {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 5, kMaxIpl = 4;
+  static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 1;
   using Params = NoParams;
   using JRec = PhiF64::JRec;
   using I = PhiF64::I;
@@ -165,7 +165,7 @@ struct AccF64 : SumOut<3> {
 };
 
 struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
-  static constexpr int kFields = 5, kMaxIpl = 4;
+  static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using JRec = Pos32cJ;
   using I = Pos32cI;
   __device__ static JRec pack(const Soa &s, int j, double sc) {
@@ -188,7 +188,7 @@ struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
 // ============================================================ AccJerk
 struct AccJerkF64 : SumOut<6> {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 8, kMaxIpl = 4;
+  static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
   using Params = NoParams;
   struct JRec { double m, x, y, z, e2, vx, vy, vz; };
   struct I { double x, y, z, e2, vx, vy, vz; };
@@ -227,7 +227,7 @@ struct AccJerkF64 : SumOut<6> {
 };
 
 struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
-  static constexpr int kFields = 8, kMaxIpl = 4;
+  static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
   // velocities stay FP64 like positions: rounding v_i to FP32 would add the
   // same error to every pair of particle i, and that coherent term
   // (dv_i * sum_j m_j/s^3) does not average out in the jerk
@@ -309,7 +309,7 @@ __device__ __forceinline__ void snap_crackle_pair(
 
 struct SnapCrackleF64 : SumOut<6> {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 14, kMaxIpl = 2;
+  static constexpr int kFields = 14, kMaxIpl = 2, kPrefIpl = 1;
   using Params = NoParams;
   struct JRec { double m, x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
   struct I { double x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
@@ -340,7 +340,7 @@ struct SnapCrackleF64 : SumOut<6> {
 };
 
 struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
-  static constexpr int kFields = 14, kMaxIpl = 2;
+  static constexpr int kFields = 14, kMaxIpl = 2, kPrefIpl = 1;
   // all relative quantities are formed in FP64 and rounded once (see
   // AccJerkF32c)
   struct JRec {
@@ -384,7 +384,7 @@ struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
 // eta/sqrt(1 + .) of each (tstep_kernel_common.h:41-55, tstep_kernel.c:46-47).
 struct TstepF64 {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 8, kMaxIpl = 2, kOut = 2;
+  static constexpr int kFields = 8, kMaxIpl = 2, kPrefIpl = 1, kOut = 2;
   struct Params { double eta; };
   using JRec = AccJerkF64::JRec;
   struct I { double m, x, y, z, e2, vx, vy, vz; };

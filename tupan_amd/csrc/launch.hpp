@@ -19,7 +19,7 @@ struct LaunchShape {
   int splits;  // j-range splits
 };
 LaunchShape choose_shape(int kernel_id, int mode, size_t ni, size_t nj,
-                         int max_ipl);
+                         int pref_ipl, int max_ipl);
 int device_alloc(void **p, size_t bytes, hipStream_t st);
 void device_free(void *p, hipStream_t st);
 // Optional live timing of the pair-engine launches (tupan_set_timing):
@@ -69,7 +69,8 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     out.f[k] = op[k];
   }
   const int n_i = (int)ni, n_j = (int)nj;
-  LaunchShape sh = choose_shape(kernel_id, mode, ni, nj, Pol::kMaxIpl);
+  LaunchShape sh = choose_shape(kernel_id, mode, ni, nj, Pol::kPrefIpl,
+                                Pol::kMaxIpl);
   const int P = sh.ipl;
   int S = sh.splits;
   int jchunk = n_j;

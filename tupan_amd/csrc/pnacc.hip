@@ -372,7 +372,7 @@ __device__ __forceinline__ PnAB pn_pair(
 
 struct PnAccF64 : SumOut<3> {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 8, kMaxIpl = 1;
+  static constexpr int kFields = 8, kMaxIpl = 1, kPrefIpl = 1;
   using Params = PnParams;
   struct JRec { double m, x, y, z, e2, vx, vy, vz; };
   using I = JRec;

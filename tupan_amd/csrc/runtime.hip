@@ -56,23 +56,25 @@ static int simd_count() {
   return cached;
 }
 
+// Launch shape: i-particles per lane P (the policy's preferred value unless
+// overridden) and j splits S.  The pair loop's scalar-load latency is hidden
+// by resident waves, so the grid aims at ~64 waves per SIMD over its
+// lifetime (measured best on MI355X for N = 16K..64K), never splitting j
+// below 128 records per wave.
 LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
-                         int max_ipl) {
+                         int pref_ipl, int max_ipl) {
   LaunchShape o{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_mu);
     if (kid >= 0 && kid < 8 && (mode == 0 || mode == 1)) o = g_override[kid][mode];
   }
-  // Aim for 4 resident waves per SIMD; never split j below 256 per wave.
-  const long target = 4L * simd_count();
-  const long max_split = nj >= 512 ? (long)(nj / 256) : 1;
-  int P = o.ipl > 0 ? o.ipl : max_ipl;
+  const long target = 64L * simd_count();
+  const long max_split = nj >= 256 ? (long)(nj / 128) : 1;
+  int P = o.ipl > 0 ? o.ipl : pref_ipl;
   if (P > max_ipl) P = max_ipl;
   if (P != 1 && P != 2 && P != 4) P = 1;
-  auto waves_i = [&](int p) { return (long)((ni + 64 * p - 1) / (64 * p)); };
-  if (o.ipl <= 0)
-    while (P > 1 && waves_i(P) * max_split < target) P /= 2;
-  long S = o.splits > 0 ? o.splits : (target + waves_i(P) - 1) / waves_i(P);
+  const long waves_i = (long)((ni + 64 * P - 1) / (64 * P));
+  long S = o.splits > 0 ? o.splits : (target + waves_i - 1) / waves_i;
   if (S > max_split && o.splits <= 0) S = max_split;
   if (S < 1) S = 1;
   if (nj > 0 && S > (long)nj) S = (long)nj;

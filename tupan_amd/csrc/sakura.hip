@@ -21,7 +21,7 @@ struct SakuraParams {
 
 struct SakuraF64 : SumOut<6> {
   static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 8, kMaxIpl = 1;
+  static constexpr int kFields = 8, kMaxIpl = 1, kPrefIpl = 1;
   using Params = SakuraParams;
   struct JRec { double m, x, y, z, e2, vx, vy, vz; };
   using I = JRec;
