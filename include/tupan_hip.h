@@ -147,6 +147,51 @@ int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
                       const double *const *jp, double dt, int flag,
                       double *const *op);
 
+/* ---- integrator updates (device pointers, FP64, asynchronous) --------
+ * The reference performs these with numpy on SoA arrays; each entry point
+ * evaluates the same expression in the same order, one IEEE operation at a
+ * time (no FMA contraction), so a device-resident driver reproduces the
+ * numpy driver's bits given the same pair-kernel outputs.
+ *
+ * Hermite predictor, replaces H2/H4/H6/H8.epredict
+ * (tupan/integrator/hermite.py:25-41, 75-90, 127-159, 203-241).
+ * x = {rx,ry,rz, vx,vy,vz, ax,ay,az, jx,jy,jz, sx,sy,sz, cx,cy,cz}; the
+ * first 6 + 3*order/2 are used.  On entry x holds the step's state and its
+ * derivatives; they are copied to x0 (the reference's ps0 = ps.copy()) and
+ * x's r, v are advanced.  order: 2, 4, 6 or 8. */
+int tupan_hermite_predict_dev(int order, size_t n, double *const *x,
+                              double *const *x0, double tau, void *stream);
+/* Hermite corrector, replaces H*.ecorrect (hermite.py:43-57, 93-119,
+ * 162-195, 244-284): x holds the predicted r, v (overwritten) and the
+ * derivatives evaluated there; x0 the snapshot written by the predictor. */
+int tupan_hermite_correct_dev(int order, size_t n, double *const *x,
+                              const double *const *x0, double tau,
+                              void *stream);
+/* drift_n / drift_pn (tupan/integrator/sia.py:65-103) and the Sakura half
+ * drifts (tupan/integrator/sakura.py:26-28, 45-47):
+ * r += v*tau (half = 0) or r += v*tau/2 (half = 1); when pn_mr is not
+ * NULL also pn_mr += pn_mv*tau (body.py:481-492). */
+int tupan_drift_dev(size_t n, double *const *r, const double *const *v,
+                    double tau, int half, double *const *pn_mr,
+                    const double *const *pn_mv, void *stream);
+/* kick_n (sia.py:79-87): v += a*tau. */
+int tupan_kick_dev(size_t n, double *const *v, const double *const *a,
+                   double tau, void *stream);
+/* half of kick_pn (sia.py:142-144, 154-156): v += (a*tau + w)/2. */
+int tupan_kick_pn_half_dev(size_t n, double *const *v, const double *const *a,
+                           const double *const *w, double tau, void *stream);
+/* middle of kick_pn (sia.py:146-152 with body.py:471-525):
+ * in  = {m, rx,ry,rz, vx,vy,vz, pnax,pnay,pnaz};
+ * acc = {wx,wy,wz, pn_ke, pn_mvx,pn_mvy,pn_mvz, pn_amx,pn_amy,pn_amz},
+ * updated: pn_ke -= (v . m pna) tau, pn_mv -= m pna tau,
+ * pn_am -= (r x m pna) tau, w = 2 pna tau - w. */
+int tupan_pn_flux_dev(size_t n, const double *const *in, double *const *acc,
+                      double tau, void *stream);
+/* sakura_step's application of the pair deltas (sakura.py:31-36, 39-44):
+ * x = {rx..vz} += d = {drx..dvz}. */
+int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
+                          void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,14 @@ def lib():
             L.tupan_sakura_host.argtypes = [_SZ, _PP, _SZ, _PP,
                                             ctypes.c_double, ctypes.c_int,
                                             _PP]
+        _D, _I = ctypes.c_double, ctypes.c_int
+        L.tupan_hermite_predict_dev.argtypes = [_I, _SZ, _PP, _PP, _D, _P]
+        L.tupan_hermite_correct_dev.argtypes = [_I, _SZ, _PP, _PP, _D, _P]
+        L.tupan_drift_dev.argtypes = [_SZ, _PP, _PP, _D, _I, _PP, _PP, _P]
+        L.tupan_kick_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
+        L.tupan_kick_pn_half_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
+        L.tupan_pn_flux_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
+        L.tupan_apply_delta_dev.argtypes = [_SZ, _PP, _PP, _P]
         _lib = L
         return _lib
 
@@ -194,6 +202,18 @@ def call_dev(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=(), stream=0,
     rc = fn(*args)
     del keep
     check(rc)
+
+
+def call_update(name, *args):
+    """Per-particle update kernel ``tupan_<name>_dev``: field-pointer lists
+    become ``double**`` (None -> NULL), the rest passes through."""
+    conv = []
+    for a in args:
+        if isinstance(a, (list, tuple)):
+            conv.append(ptr_array(a))
+        else:
+            conv.append(a)
+    check(getattr(lib(), "tupan_%s_dev" % name)(*conv))
 
 
 def as_f64(a):

@@ -1,0 +1,398 @@
+// integrate.hip -- per-particle update steps of Tupan's integrators, FP64.
+//
+// The reference performs these updates with numpy expressions on SoA
+// arrays; every kernel here evaluates the same expression in the same order,
+// one IEEE operation at a time (no contraction into FMA: `fp contract(off)`;
+// FP64 division stays correctly rounded), so that a device-resident driver
+// produces the numpy driver's bits given the same pair-kernel outputs.
+//
+//   hermite_predict  tupan/integrator/hermite.py:25-41 (H2), 75-90 (H4),
+//                    127-159 (H6), 203-241 (H8)
+//   hermite_correct  hermite.py:43-57 (H2), 93-119 (H4), 162-195 (H6),
+//                    244-284 (H8)
+//   drift / kick     tupan/integrator/sia.py:65-87 (drift_n, kick_n),
+//                    sakura.py:26-28, 45-47 (half drifts)
+//   kick_pn          sia.py:142-156 with tupan/particles/body.py:471-525
+//                    (pn_kick_ke, pn_kick_lmom, pn_kick_amom) and
+//                    body.py:481-492 (pn_drift_com_r)
+//   apply_delta      sakura.py:31-36, 39-44
+#include "launch.hpp"
+
+namespace tupan {
+namespace {
+
+constexpr int kThreads = 256;
+
+struct P18 {
+  double *f[18];
+};
+
+inline int grid_for(size_t n) {
+  size_t g = (n + kThreads - 1) / kThreads;
+  return (int)(g < 65535 * 8 ? g : 65535 * 8);
+}
+
+// x = {rx,ry,rz, vx,vy,vz, ax,ay,az, jx,jy,jz, sx,sy,sz, cx,cy,cz}
+// On entry x holds the step's initial state and its derivatives; x0
+// receives that snapshot (the reference's ps0 = ps.copy()), then x's
+// positions and velocities are advanced by the Taylor predictor.
+template <int ORDER>
+__global__ void __launch_bounds__(kThreads)
+    hermite_predict_kernel(int n, P18 x, P18 x0, double tau) {
+#pragma clang fp contract(off)
+  constexpr int nd = 6 + 3 * (ORDER / 2);  // r, v + derivatives used
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+    double s[18];
+#pragma unroll
+    for (int k = 0; k < nd; ++k) {
+      s[k] = x.f[k][i];
+      x0.f[k][i] = s[k];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double r = s[d], v = s[3 + d], a = s[6 + d];
+      double dr, dv;
+      if (ORDER == 2) {
+        dr = (a * tau / 2 + v) * tau;
+        dv = a * tau;
+      } else if (ORDER == 4) {
+        const double j = s[9 + d];
+        dr = ((j * tau / 3 + a) * tau / 2 + v) * tau;
+        dv = (j * tau / 2 + a) * tau;
+      } else if (ORDER == 6) {
+        const double j = s[9 + d], sn = s[12 + d];
+        dr = (((sn * tau / 4 + j) * tau / 3 + a) * tau / 2 + v) * tau;
+        dv = ((sn * tau / 3 + j) * tau / 2 + a) * tau;
+      } else {
+        const double j = s[9 + d], sn = s[12 + d], c = s[15 + d];
+        dr = ((((c * tau / 5 + sn) * tau / 4 + j) * tau / 3 + a) * tau / 2 +
+              v) * tau;
+        dv = (((c * tau / 4 + sn) * tau / 3 + j) * tau / 2 + a) * tau;
+      }
+      x.f[d][i] = r + dr;
+      x.f[3 + d][i] = v + dv;
+    }
+  }
+}
+
+// x: predicted r, v (overwritten with the corrected ones) and the
+// derivatives evaluated at the predicted state; x0: the step's snapshot.
+template <int ORDER>
+__global__ void __launch_bounds__(kThreads)
+    hermite_correct_kernel(int n, P18 x, P18 x0, double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double r0 = x0.f[d][i], v0 = x0.f[3 + d][i];
+      const double a0 = x0.f[6 + d][i], a1 = x.f[6 + d][i];
+      double v1, r1;
+      if (ORDER == 2) {
+        v1 = (a0 + a1) * tau / 2 + v0;
+        r1 = (v0 + v1) * tau / 2 + r0;
+      } else if (ORDER == 4) {
+        const double j0 = x0.f[9 + d][i], j1 = x.f[9 + d][i];
+        v1 = ((j0 - j1) * tau / 6 + (a0 + a1)) * tau / 2 + v0;
+        r1 = ((a0 - a1) * tau / 6 + (v0 + v1)) * tau / 2 + r0;
+      } else if (ORDER == 6) {
+        const double j0 = x0.f[9 + d][i], j1 = x.f[9 + d][i];
+        const double s0 = x0.f[12 + d][i], s1 = x.f[12 + d][i];
+        v1 = (((s0 + s1) * tau / 12 + (j0 - j1)) * tau / 5 + (a0 + a1)) *
+                 tau / 2 +
+             v0;
+        r1 = (((j0 + j1) * tau / 12 + (a0 - a1)) * tau / 5 + (v0 + v1)) *
+                 tau / 2 +
+             r0;
+      } else {
+        const double j0 = x0.f[9 + d][i], j1 = x.f[9 + d][i];
+        const double s0 = x0.f[12 + d][i], s1 = x.f[12 + d][i];
+        const double c0 = x0.f[15 + d][i], c1 = x.f[15 + d][i];
+        v1 = ((((c0 - c1) * tau / 20 + (s0 + s1)) * tau / 3 +
+               3 * (j0 - j1)) * tau / 14 +
+              (a0 + a1)) * tau / 2 +
+             v0;
+        r1 = ((((s0 - s1) * tau / 20 + (j0 + j1)) * tau / 3 +
+               3 * (a0 - a1)) * tau / 14 +
+              (v0 + v1)) * tau / 2 +
+             r0;
+      }
+      x.f[3 + d][i] = v1;
+      x.f[d][i] = r1;
+    }
+  }
+}
+
+// r += v * tau (half = 0) or r += v * tau / 2 (half = 1); with PN
+// bookkeeping also pn_mr += pn_mv * tau.
+__global__ void __launch_bounds__(kThreads)
+    drift_kernel(int n, P18 p, double tau, int half, int pn) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double dr = p.f[3 + d][i] * tau;
+      if (half) dr = dr / 2;
+      p.f[d][i] = p.f[d][i] + dr;
+      if (pn) p.f[6 + d][i] = p.f[6 + d][i] + p.f[9 + d][i] * tau;
+    }
+  }
+}
+
+// v += a * tau
+__global__ void __launch_bounds__(kThreads)
+    kick_kernel(int n, P18 p, double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p.f[d][i] = p.f[d][i] + p.f[3 + d][i] * tau;
+  }
+}
+
+// v += (a * tau + w) / 2
+__global__ void __launch_bounds__(kThreads)
+    kick_pn_half_kernel(int n, P18 p, double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      p.f[d][i] = p.f[d][i] + (p.f[3 + d][i] * tau + p.f[6 + d][i]) / 2;
+  }
+}
+
+// In  p.f[0..9]  = m, rx, ry, rz, vx, vy, vz, pnax, pnay, pnaz
+// Out p.f[10..17]= wx, wy, wz, pn_ke, pn_mvx..z (in/out);
+// q.f[0..2]      = pn_amx..z (in/out)
+__global__ void __launch_bounds__(kThreads)
+    pn_flux_kernel(int n, P18 p, P18 q, double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+    const double m = p.f[0][i];
+    const double rx = p.f[1][i], ry = p.f[2][i], rz = p.f[3][i];
+    const double vx = p.f[4][i], vy = p.f[5][i], vz = p.f[6][i];
+    const double ax = p.f[7][i], ay = p.f[8][i], az = p.f[9][i];
+    const double fx = m * ax, fy = m * ay, fz = m * az;
+    // pn_kick_ke
+    p.f[13][i] = p.f[13][i] - (vx * fx + vy * fy + vz * fz) * tau;
+    // pn_kick_lmom
+    p.f[14][i] = p.f[14][i] - fx * tau;
+    p.f[15][i] = p.f[15][i] - fy * tau;
+    p.f[16][i] = p.f[16][i] - fz * tau;
+    // pn_kick_amom
+    q.f[0][i] = q.f[0][i] - (ry * fz - rz * fy) * tau;
+    q.f[1][i] = q.f[1][i] - (rz * fx - rx * fz) * tau;
+    q.f[2][i] = q.f[2][i] - (rx * fy - ry * fx) * tau;
+    // w = 2 * pna * tau - w
+    p.f[10][i] = 2 * ax * tau - p.f[10][i];
+    p.f[11][i] = 2 * ay * tau - p.f[11][i];
+    p.f[12][i] = 2 * az * tau - p.f[12][i];
+  }
+}
+
+// x += d over six fields (r, v)
+__global__ void __launch_bounds__(kThreads)
+    apply_delta_kernel(int n, P18 p) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) p.f[k][i] = p.f[k][i] + p.f[6 + k][i];
+  }
+}
+
+int check_fields(const void *const *f, int k, const char *what) {
+  if (!f) return set_error(TUPAN_ERR_ARG, "NULL %s field array", what);
+  for (int i = 0; i < k; ++i)
+    if (!f[i]) return set_error(TUPAN_ERR_ARG, "NULL %s field %d", what, i);
+  return TUPAN_OK;
+}
+
+int check_n(size_t n) {
+  if (n > (size_t)INT_MAX)
+    return set_error(TUPAN_ERR_ARG, "n too large (%zu)", n);
+  return TUPAN_OK;
+}
+
+int finish(const char *what) {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TUPAN_OK : hip_fail(e, what);
+}
+
+template <class K>
+int launch_order(int order, K &&k) {
+  switch (order) {
+    case 2: k(std::integral_constant<int, 2>{}); break;
+    case 4: k(std::integral_constant<int, 4>{}); break;
+    case 6: k(std::integral_constant<int, 6>{}); break;
+    case 8: k(std::integral_constant<int, 8>{}); break;
+    default:
+      return set_error(TUPAN_ERR_ARG, "Hermite order must be 2, 4, 6 or 8 "
+                                      "(got %d)", order);
+  }
+  return TUPAN_OK;
+}
+
+}  // namespace
+}  // namespace tupan
+
+using namespace tupan;
+
+extern "C" {
+
+int tupan_hermite_predict_dev(int order, size_t n, double *const *x,
+                              double *const *x0, double tau, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  const int nf = 6 + 3 * (order / 2);
+  if (order == 2 || order == 4 || order == 6 || order == 8) {
+    if ((rc = check_fields((const void *const *)x, nf, "state"))) return rc;
+    if ((rc = check_fields((const void *const *)x0, nf, "snapshot")))
+      return rc;
+  }
+  if (n == 0) return launch_order(order, [](auto) {});
+  P18 a{}, b{};
+  for (int k = 0; k < nf && k < 18; ++k) {
+    a.f[k] = x[k];
+    b.f[k] = x0[k];
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  rc = launch_order(order, [&](auto o) {
+    hipLaunchKernelGGL((hermite_predict_kernel<decltype(o)::value>),
+                       dim3(grid_for(n)), dim3(kThreads), 0, st, (int)n, a, b,
+                       tau);
+  });
+  return rc ? rc : finish("hermite_predict");
+}
+
+int tupan_hermite_correct_dev(int order, size_t n, double *const *x,
+                              const double *const *x0, double tau,
+                              void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  const int nf = 6 + 3 * (order / 2);
+  if (order == 2 || order == 4 || order == 6 || order == 8) {
+    if ((rc = check_fields((const void *const *)x, nf, "state"))) return rc;
+    if ((rc = check_fields((const void *const *)x0, nf, "snapshot")))
+      return rc;
+  }
+  if (n == 0) return launch_order(order, [](auto) {});
+  P18 a{}, b{};
+  for (int k = 0; k < nf && k < 18; ++k) {
+    a.f[k] = x[k];
+    b.f[k] = const_cast<double *>(x0[k]);
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  rc = launch_order(order, [&](auto o) {
+    hipLaunchKernelGGL((hermite_correct_kernel<decltype(o)::value>),
+                       dim3(grid_for(n)), dim3(kThreads), 0, st, (int)n, a, b,
+                       tau);
+  });
+  return rc ? rc : finish("hermite_correct");
+}
+
+int tupan_drift_dev(size_t n, double *const *r, const double *const *v,
+                    double tau, int half, double *const *pn_mr,
+                    const double *const *pn_mv, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)r, 3, "position"))) return rc;
+  if ((rc = check_fields((const void *const *)v, 3, "velocity"))) return rc;
+  const int pn = pn_mr != nullptr;
+  if (pn) {
+    if ((rc = check_fields((const void *const *)pn_mr, 3, "pn_mr"))) return rc;
+    if ((rc = check_fields((const void *const *)pn_mv, 3, "pn_mv"))) return rc;
+  }
+  if (n == 0) return TUPAN_OK;
+  P18 p{};
+  for (int k = 0; k < 3; ++k) {
+    p.f[k] = r[k];
+    p.f[3 + k] = const_cast<double *>(v[k]);
+    if (pn) {
+      p.f[6 + k] = pn_mr[k];
+      p.f[9 + k] = const_cast<double *>(pn_mv[k]);
+    }
+  }
+  hipLaunchKernelGGL(drift_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, p, tau,
+                     half ? 1 : 0, pn);
+  return finish("drift");
+}
+
+int tupan_kick_dev(size_t n, double *const *v, const double *const *a,
+                   double tau, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)v, 3, "velocity"))) return rc;
+  if ((rc = check_fields((const void *const *)a, 3, "acceleration")))
+    return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{};
+  for (int k = 0; k < 3; ++k) {
+    p.f[k] = v[k];
+    p.f[3 + k] = const_cast<double *>(a[k]);
+  }
+  hipLaunchKernelGGL(kick_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, p, tau);
+  return finish("kick");
+}
+
+int tupan_kick_pn_half_dev(size_t n, double *const *v, const double *const *a,
+                           const double *const *w, double tau, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)v, 3, "velocity"))) return rc;
+  if ((rc = check_fields((const void *const *)a, 3, "acceleration")))
+    return rc;
+  if ((rc = check_fields((const void *const *)w, 3, "w"))) return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{};
+  for (int k = 0; k < 3; ++k) {
+    p.f[k] = v[k];
+    p.f[3 + k] = const_cast<double *>(a[k]);
+    p.f[6 + k] = const_cast<double *>(w[k]);
+  }
+  hipLaunchKernelGGL(kick_pn_half_kernel, dim3(grid_for(n)), dim3(kThreads),
+                     0, reinterpret_cast<hipStream_t>(stream), (int)n, p, tau);
+  return finish("kick_pn_half");
+}
+
+int tupan_pn_flux_dev(size_t n, const double *const *in, double *const *acc,
+                      double tau, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)in, 10, "input"))) return rc;
+  if ((rc = check_fields((const void *const *)acc, 10, "accumulator")))
+    return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{}, q{};
+  for (int k = 0; k < 10; ++k) p.f[k] = const_cast<double *>(in[k]);
+  for (int k = 0; k < 7; ++k) p.f[10 + k] = acc[k];
+  for (int k = 0; k < 3; ++k) q.f[k] = acc[7 + k];
+  hipLaunchKernelGGL(pn_flux_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, p, q, tau);
+  return finish("pn_flux");
+}
+
+int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
+                          void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)x, 6, "state"))) return rc;
+  if ((rc = check_fields((const void *const *)d, 6, "delta"))) return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{};
+  for (int k = 0; k < 6; ++k) {
+    p.f[k] = x[k];
+    p.f[6 + k] = const_cast<double *>(d[k]);
+  }
+  hipLaunchKernelGGL(apply_delta_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, p);
+  return finish("apply_delta");
+}
+
+}  // extern "C"
