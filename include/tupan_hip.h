@@ -59,6 +59,9 @@ extern "C" {
 #define TUPAN_K_TSTEP 4
 #define TUPAN_K_PNACC 5
 #define TUPAN_K_SAKURA 6
+#define TUPAN_K_ACC_ADJOINT 7
+#define TUPAN_K_PHI_ADJOINT 8
+#define TUPAN_K_COUNT 9
 
 /* ---- runtime ------------------------------------------------------- */
 const char *tupan_version(void);
@@ -146,6 +149,30 @@ int tupan_sakura_dev(size_t ni, const double *const *ip, size_t nj,
 int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
                       const double *const *jp, double dt, int flag,
                       double *const *op);
+
+/* ---- adjoints of Acc and Phi ------------------------------------------
+ * No counterpart in the reference (it has no differentiation).  For a
+ * self-gravitating set (i = j = all particles) with a = acc(ps, ps) or
+ * phi = phi(ps, ps) and a loss L, given the upstream gradient
+ * g_k = dL/da_k (Acc) or h_k = dL/dphi_k (Phi) these return the full
+ * gradients with respect to every position and mass:
+ *
+ *   Acc:  dL/dx_k = sum_j M(r_kj) (m_k g_j - m_j g_k),
+ *         M(r) = I/s^3 - 3 r r^T/s^5 (the softened tidal tensor, negated),
+ *         dL/dm_k = sum_j g_j . r_kj / s_kj^3
+ *   Phi:  dL/dx_k = sum_j (h_k m_j + h_j m_k) r_kj / s_kj^3,
+ *         dL/dm_k = -sum_j h_j / s_kj
+ *
+ * with r_kj = x_k - x_j, s^2 = r^2 + e2_k + e2_j and the reference's r^2 > 0
+ * mask.  Both are pair sums on the same engine and modes as Acc/Phi.
+ * ip, jp: Acc {m, rx, ry, rz, e2, gx, gy, gz}; Phi {m, rx, ry, rz, e2, h}.
+ * op: dL/drx, dL/dry, dL/drz, dL/dm. */
+int tupan_acc_adjoint_dev(int mode, size_t ni, const double *const *ip,
+                          size_t nj, const double *const *jp,
+                          double *const *op, void *stream);
+int tupan_phi_adjoint_dev(int mode, size_t ni, const double *const *ip,
+                          size_t nj, const double *const *jp,
+                          double *const *op, void *stream);
 
 /* ---- integrator updates (device pointers, FP64, asynchronous) --------
  * The reference performs these with numpy on SoA arrays; each entry point

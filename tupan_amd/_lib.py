@@ -73,9 +73,11 @@ def lib():
         L.tupan_set_sakura_budget.argtypes = [ctypes.c_int]
         L.tupan_take_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int)]
-        for name in ("phi", "acc", "acc_jerk", "snap_crackle"):
+        for name in ("phi", "acc", "acc_jerk", "snap_crackle", "acc_adjoint",
+                     "phi_adjoint"):
             getattr(L, "tupan_%s_dev" % name).argtypes = [
                 ctypes.c_int, _SZ, _PP, _SZ, _PP, _PP, _P]
+        for name in ("phi", "acc", "acc_jerk", "snap_crackle"):
             getattr(L, "tupan_%s_host" % name).argtypes = [
                 ctypes.c_int, _SZ, _PP, _SZ, _PP, _PP]
         L.tupan_tstep_dev.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_double,

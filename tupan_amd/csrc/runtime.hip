@@ -27,7 +27,7 @@ int hip_fail(hipError_t e, const char *what) {
 
 // ------------------------------------------------------ launch shapes
 static std::mutex g_mu;
-static LaunchShape g_override[8][2];  // [kernel][mode], zero = automatic
+static LaunchShape g_override[TUPAN_K_COUNT][2];  // [kernel][mode], zero = automatic
 static int g_default_mode = TUPAN_MODE_FP64;
 static double g_near_c = 2.0;          // near-field radius, see engine.hpp
 static int g_sakura_budget = 1 << 20;  // Kepler solves per Sakura pair
@@ -66,7 +66,8 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
   LaunchShape o{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (kid >= 0 && kid < 8 && (mode == 0 || mode == 1)) o = g_override[kid][mode];
+    if (kid >= 0 && kid < TUPAN_K_COUNT && (mode == 0 || mode == 1))
+      o = g_override[kid][mode];
   }
   const long target = 64L * simd_count();
   const long max_split = nj >= 256 ? (long)(nj / 128) : 1;
@@ -256,7 +257,7 @@ int tupan_take_kernel_times(double *total_ms, int *count) {
 }
 
 int tupan_set_launch(int kernel_id, int mode, int ipl, int splits) {
-  if (kernel_id < 0 || kernel_id >= 8)
+  if (kernel_id < 0 || kernel_id >= TUPAN_K_COUNT)
     return set_error(TUPAN_ERR_ARG, "bad kernel id %d", kernel_id);
   if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
     return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);

@@ -28,6 +28,14 @@ KERNELS = {
     "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), False, None),
     "sakura": Kernel("sakura", 6, F8,
                      ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
+    # reverse-mode derivatives (no reference counterpart): inputs F5 plus the
+    # upstream gradient dL/da (gx, gy, gz) or dL/dphi (h); outputs dL/dr and
+    # dL/dm.  Flops per pair count every IEEE operation of the FP64 pair
+    # (rsq as one).
+    "acc_adjoint": Kernel("acc_adjoint", 7, F5 + ("gx", "gy", "gz"),
+                          ("grx", "gry", "grz", "gm"), True, 48),
+    "phi_adjoint": Kernel("phi_adjoint", 8, F5 + ("h",),
+                          ("grx", "gry", "grz", "gm"), True, 25),
 }
 
 KERNEL_IDS = {k: v.kid for k, v in KERNELS.items()}
