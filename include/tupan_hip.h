@@ -174,6 +174,42 @@ int tupan_phi_adjoint_dev(int mode, size_t ni, const double *const *ip,
                           size_t nj, const double *const *jp,
                           double *const *op, void *stream);
 
+/* ---- ragged ensemble of independent clusters --------------------------
+ * No single counterpart in the reference: each cluster follows the
+ * reference's adaptive Hermite-4 ("ahermite4", tupan/integrator/
+ * hermite.py:340-410 with the block steps of tupan/integrator/
+ * __init__.py:48-78) on its own clock, and the ensemble batches the pair
+ * sums of all clusters into one persistent kernel per phase.  One ensemble
+ * step (Tstep, block-step choice, AccJerk + predict, 2 x (AccJerk +
+ * correct), clock) is captured once into a HIP graph and replayed.
+ * FP64 arithmetic throughout.  Device memory is allocated at create and at
+ * the first step after set_launch; steps allocate nothing. */
+typedef struct tupan_ensemble tupan_ensemble;
+/* sizes[c] = particles of cluster c; clusters are stored back to back. */
+int tupan_ensemble_create(tupan_ensemble **out, int nclusters,
+                          const int *sizes, double eta, double t_end);
+int tupan_ensemble_destroy(tupan_ensemble *e);
+/* Autotuner hook: i-particles per lane (1, 2), j-chunk length of a work
+ * item, resident waves per SIMD of the persistent kernels. */
+int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
+                              int waves_per_simd);
+/* fields = host {m, rx, ry, rz, e2, vx, vy, vz}, each of the total length;
+ * t0 = per-cluster start times (NULL = 0).  Synchronous. */
+int tupan_ensemble_set_state(tupan_ensemble *e, const double *const *fields,
+                             const double *t0);
+/* out = host {rx, ry, rz, vx, vy, vz} (entries may be NULL); tc, nsteps =
+ * per-cluster clocks and step counts (may be NULL).  Synchronous. */
+int tupan_ensemble_get_state(tupan_ensemble *e, double *const *out,
+                             double *tc, long long *nsteps);
+/* Queue `steps` ensemble steps (graph = 1: replay the captured graph;
+ * 0: launch the same kernels one by one).  Finished clusters idle. */
+int tupan_ensemble_step(tupan_ensemble *e, int steps, int graph);
+/* Wait for queued steps; unfinished clusters, pair interactions evaluated
+ * since set_state, error flag (1 = a cluster met a zero or non-finite time
+ * step and was stopped). */
+int tupan_ensemble_poll(tupan_ensemble *e, int *unfinished, double *pairs,
+                        int *error);
+
 /* ---- integrator updates (device pointers, FP64, asynchronous) --------
  * The reference performs these with numpy on SoA arrays; each entry point
  * evaluates the same expression in the same order, one IEEE operation at a

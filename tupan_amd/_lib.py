@@ -104,6 +104,17 @@ def lib():
         L.tupan_kick_pn_half_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
         L.tupan_pn_flux_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
         L.tupan_apply_delta_dev.argtypes = [_SZ, _PP, _PP, _P]
+        _PI = ctypes.POINTER(ctypes.c_int)
+        _PD = ctypes.POINTER(ctypes.c_double)
+        L.tupan_ensemble_create.argtypes = [ctypes.POINTER(_P), _I, _PI, _D,
+                                            _D]
+        L.tupan_ensemble_destroy.argtypes = [_P]
+        L.tupan_ensemble_set_launch.argtypes = [_P, _I, _I, _I]
+        L.tupan_ensemble_set_state.argtypes = [_P, _PP, _PD]
+        L.tupan_ensemble_get_state.argtypes = [
+            _P, _PP, _PD, ctypes.POINTER(ctypes.c_longlong)]
+        L.tupan_ensemble_step.argtypes = [_P, _I, _I]
+        L.tupan_ensemble_poll.argtypes = [_P, _PI, _PD, _PI]
         _lib = L
         return _lib
 

@@ -1,0 +1,747 @@
+// ensemble.hip -- a ragged ensemble of independent clusters integrated with
+// adaptive Hermite-4 in one device-resident loop.
+//
+// Every cluster c (n_c particles, stored contiguously from off_c) follows
+// exactly the reference's adaptive Hermite-4 step ("ahermite4",
+// tupan/integrator/hermite.py:340-410 with the block steps of
+// tupan/integrator/__init__.py:48-78) on its own clock t_c:
+//
+//   tstep  <- Tstep(c, c, eta)                      (FP64 pair sums)
+//   tau    <- block step from min |tstep| and min(t_end - t_c, eta)
+//   snapshot, a0, j0 <- AccJerk(c, c); predict
+//   twice: a1, j1 <- AccJerk(c, c); correct
+//   t_c += tau
+//
+// Clusters do not interact, so they need no common time line: one "step"
+// of the ensemble advances every unfinished cluster by one of its own steps.
+// The step is a fixed sequence of kernels that read each cluster's state
+// (active flag, tau) from device memory, so it is captured once into a HIP
+// graph and replayed; the host only polls how many clusters are unfinished.
+//
+// Pair sums: a persistent kernel walks a static list of work items
+// (cluster, i-wave, j-chunk), sorted by cost and dealt round-robin to the
+// resident waves, so that 64-particle and 4096-particle clusters share the
+// chip evenly.  Each item writes FP64 partial sums; a finishing kernel adds a
+// cluster's chunks in chunk order (bit-reproducible).  The pair arithmetic
+// is the single-evaluation FP64 policies' (gravity.hip).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "hermite.hpp"
+#include "launch.hpp"
+
+namespace tupan {
+
+// j record of the AccJerk and Tstep pair terms (the reference's argument
+// order, tupan/lib/src/libtupan.h:34-58)
+struct AccJerkF64Rec { double m, x, y, z, e2, vx, vy, vz; };
+
+namespace {
+
+constexpr int kThreads = 256;
+
+struct ClusterInfo {
+  int off, n, nchunk;
+  long long base;  // offset of the cluster's partials (doubles)
+};
+
+struct Item {
+  int c, i0, j0, j1, chunk;
+};
+
+// ---------------------------------------------------------------- pairs
+// FP64 AccJerk and Tstep pair terms, as in gravity.hip's AccJerkF64 and
+// TstepF64 (acc_jerk_kernel_common.h:7-55, tstep_kernel_common.h:7-56).
+struct AJ {
+  static constexpr int kOut = 6;
+  struct A { double v[6]; };
+  struct Params {};
+  __device__ static void zero(A &a) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.v[k] = 0.0;
+  }
+  __device__ static void pair(const AccJerkF64Rec &q, const AccJerkF64Rec &r,
+                              A &a, const Params &) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double vx = q.vx - r.vx, vy = q.vy - r.vy, vz = q.vz - r.vz;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double rv = rx * vx + ry * vy + rz * vz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = r2 > 0.0 ? y : 0.0;
+    double y2 = y * y;
+    double k = r.m * y * y2;
+    double alpha = 3.0 * rv * y2;
+    vx -= alpha * rx;
+    vy -= alpha * ry;
+    vz -= alpha * rz;
+    a.v[0] -= k * rx;
+    a.v[1] -= k * ry;
+    a.v[2] -= k * rz;
+    a.v[3] -= k * vx;
+    a.v[4] -= k * vy;
+    a.v[5] -= k * vz;
+  }
+  __device__ static void values(const A &a, double *v) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = a.v[k];
+  }
+  __device__ static void combine(double *v, const double *w) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += w[k];
+  }
+  __device__ static void finish(const double *v, double *const *out, int i,
+                                const Params &) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k][i] = v[k];
+  }
+};
+
+struct TS {
+  static constexpr int kOut = 2;
+  struct A { double sum, mx; };
+  struct Params { double eta; };
+  __device__ static void zero(A &a) { a.sum = 0.0; a.mx = 0.0; }
+  __device__ static void pair(const AccJerkF64Rec &q, const AccJerkF64Rec &r,
+                              A &a, const Params &prm) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double vx = q.vx - r.vx, vy = q.vy - r.vy, vz = q.vz - r.vz;
+    double m = q.m + r.m;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double rv = rx * vx + ry * vy + rz * vz;
+    double v2 = vx * vx + vy * vy + vz * vz;
+    const bool real_pair = r2 > 0.0;
+    double inv_r1 = rsq64(r2 + (q.e2 + r.e2));
+    inv_r1 = real_pair ? inv_r1 : 0.0;
+    double inv_r2 = inv_r1 * inv_r1;
+    double phi = m * inv_r1;
+    double w2 = inv_r2 * (v2 + 2.0 * phi);
+    double gam = inv_r2 * (w2 + 2.0 * inv_r2 * phi);
+    gam *= prm.eta * rsq64(w2);
+    w2 -= gam * rv;
+    w2 = real_pair ? w2 : 0.0;
+    a.sum += w2;
+    a.mx = fmax(w2, a.mx);
+  }
+  __device__ static void values(const A &a, double *v) {
+    v[0] = a.sum;
+    v[1] = a.mx;
+  }
+  __device__ static void combine(double *v, const double *w) {
+    v[0] += w[0];
+    v[1] = fmax(w[1], v[1]);
+  }
+  __device__ static void finish(const double *v, double *const *out, int i,
+                                const Params &prm) {
+    out[0][i] = prm.eta / sqrt(1.0 + v[0]);
+    out[1][i] = prm.eta / sqrt(1.0 + v[1]);
+  }
+};
+
+struct Outs {
+  double *f[6];
+};
+
+// Persistent pair kernel.  Wave w handles items w, w + W, w + 2W, ... of
+// the cost-sorted list.  Item and j indices are made wave-uniform
+// (readfirstlane) so that item descriptors and j records are scalar loads.
+template <class Pol, int P>
+__global__ __launch_bounds__(kThreads) void ens_pairs(
+    const AccJerkF64Rec *__restrict__ rec, const Item *__restrict__ items,
+    int nitems, const ClusterInfo *__restrict__ cl,
+    const int *__restrict__ active, double *__restrict__ part,
+    typename Pol::Params prm) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
+  const int nwaves = gridDim.x * (kThreads / 64);
+  for (int it = wave; it < nitems; it += nwaves) {
+    const Item w = items[it];
+    if (!active[w.c]) continue;
+    const ClusterInfo ci = cl[w.c];
+    typename Pol::A acc[P];
+    AccJerkF64Rec q[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      int li = w.i0 + p * 64 + lane;
+      li = li < ci.n ? li : ci.n - 1;
+      q[p] = rec[ci.off + li];
+      Pol::zero(acc[p]);
+    }
+    const int jb = ci.off + w.j0, je = ci.off + w.j1;
+    AccJerkF64Rec r = rec[jb];
+#pragma unroll 2
+    for (int j = jb; j < je; ++j) {
+      const AccJerkF64Rec rn = rec[min(j + 1, je - 1)];
+#pragma unroll
+      for (int p = 0; p < P; ++p) Pol::pair(q[p], r, acc[p], prm);
+      r = rn;
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int li = w.i0 + p * 64 + lane;
+      if (li >= ci.n) continue;
+      double v[Pol::kOut];
+      Pol::values(acc[p], v);
+#pragma unroll
+      for (int k = 0; k < Pol::kOut; ++k)
+        part[ci.base + ((long long)w.chunk * Pol::kOut + k) * ci.n + li] =
+            v[k];
+    }
+  }
+}
+
+// Adds a cluster's chunk partials in chunk order and finishes.
+template <class Pol>
+__global__ __launch_bounds__(kThreads) void ens_finish(
+    int ntot, const int *__restrict__ cid, const ClusterInfo *__restrict__ cl,
+    const int *__restrict__ active, const double *__restrict__ part, Outs out,
+    typename Pol::Params prm) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < ntot;
+       i += gridDim.x * kThreads) {
+    const int c = cid[i];
+    if (!active[c]) continue;
+    const ClusterInfo ci = cl[c];
+    const int li = i - ci.off;
+    double v[Pol::kOut];
+#pragma unroll
+    for (int k = 0; k < Pol::kOut; ++k) v[k] = part[ci.base + (long long)k * ci.n + li];
+    for (int s = 1; s < ci.nchunk; ++s) {
+      double w[Pol::kOut];
+#pragma unroll
+      for (int k = 0; k < Pol::kOut; ++k)
+        w[k] = part[ci.base + ((long long)s * Pol::kOut + k) * ci.n + li];
+      Pol::combine(v, w);
+    }
+    Pol::finish(v, out.f, i, prm);
+  }
+}
+
+// Field indices of the ensemble state.
+enum EF {
+  FM = 0, FX, FY, FZ, FE2, FVX, FVY, FVZ,  // base (reference order)
+  FAX, FAY, FAZ, FJX, FJY, FJZ,            // derivatives at the state
+  S_X, S_Y, S_Z, S_VX, S_VY, S_VZ, S_AX, S_AY, S_AZ, S_JX, S_JY, S_JZ,
+  FTS, FTSIJ,                              // tstep, tstepij
+  NFIELD
+};
+
+struct Fields {
+  double *f[NFIELD];
+};
+
+__global__ __launch_bounds__(kThreads) void ens_pack(int ntot, Fields s,
+                                                     const int *cid,
+                                                     const int *active,
+                                                     AccJerkF64Rec *rec) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < ntot;
+       i += gridDim.x * kThreads) {
+    if (!active[cid[i]]) continue;
+    rec[i] = {s.f[FM][i],  s.f[FX][i],  s.f[FY][i],  s.f[FZ][i],
+              s.f[FE2][i], s.f[FVX][i], s.f[FVY][i], s.f[FVZ][i]};
+  }
+}
+
+// Block step of each active cluster from min |tstep| over its particles:
+// get_base_tstep then get_min_block_tstep (integrator/__init__.py:48-78).
+// One block per cluster; the minimum is exact, so the reduction order does
+// not matter.
+__global__ __launch_bounds__(kThreads) void ens_choose_dt(
+    const ClusterInfo *__restrict__ cl, const double *__restrict__ tstep,
+    const double *__restrict__ tc, const int *__restrict__ active,
+    double *__restrict__ dt, double eta, double t_end,
+    int *__restrict__ err) {
+  const int c = blockIdx.x;
+  if (!active[c]) return;
+  const ClusterInfo ci = cl[c];
+  __shared__ double red[kThreads];
+  double mn = INFINITY;
+  for (int i = threadIdx.x; i < ci.n; i += kThreads)
+    mn = fmin(mn, fabs(tstep[ci.off + i]));
+  red[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double t = tc[c];
+  // get_base_tstep
+  double tau = fmin(fabs(t_end) - fabs(t), fabs(eta));
+  tau = fmax(tau, fabs(t_end) * (2 * 2.220446049250313e-16));
+  tau = copysign(tau, eta);
+  // get_min_block_tstep
+  const double min_ts = red[0];
+  if (!(min_ts > 0.0) || !(min_ts < INFINITY)) {
+    // a vanishing or non-finite time step: stop this cluster
+    err[0] = 1;
+    dt[c] = 0.0;
+    return;
+  }
+  const int power = (int)(log2(min_ts) - 1.0);
+  double min_bts = ldexp(1.0, power);
+  const double t_next = t + min_bts;
+  for (int k = 0; k < 2200 && fmod(t_next, min_bts) != 0.0; ++k)
+    min_bts /= 2;
+  min_bts = copysign(min_bts, tau);
+  if (fabs(min_bts) > fabs(tau)) min_bts = tau;
+  dt[c] = min_bts;
+}
+
+__global__ __launch_bounds__(kThreads) void ens_predict(
+    int ntot, Fields s, const int *__restrict__ cid,
+    const int *__restrict__ active, const double *__restrict__ dt) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < ntot;
+       i += gridDim.x * kThreads) {
+    const int c = cid[i];
+    if (!active[c]) continue;
+    const double tau = dt[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double r = s.f[FX + d][i], v = s.f[FVX + d][i];
+      const double a = s.f[FAX + d][i], j = s.f[FJX + d][i];
+      s.f[S_X + d][i] = r;
+      s.f[S_VX + d][i] = v;
+      s.f[S_AX + d][i] = a;
+      s.f[S_JX + d][i] = j;
+      double r1, v1;
+      hermite_predict_axis<4>(r, v, a, j, 0.0, 0.0, tau, r1, v1);
+      s.f[FX + d][i] = r1;
+      s.f[FVX + d][i] = v1;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void ens_correct(
+    int ntot, Fields s, const int *__restrict__ cid,
+    const int *__restrict__ active, const double *__restrict__ dt) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < ntot;
+       i += gridDim.x * kThreads) {
+    const int c = cid[i];
+    if (!active[c]) continue;
+    const double tau = dt[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double r1, v1;
+      hermite_correct_axis<4>(s.f[S_X + d][i], s.f[S_VX + d][i],
+                              s.f[S_AX + d][i], s.f[FAX + d][i],
+                              s.f[S_JX + d][i], s.f[FJX + d][i], 0.0, 0.0,
+                              0.0, 0.0, tau, r1, v1);
+      s.f[FVX + d][i] = v1;
+      s.f[FX + d][i] = r1;
+    }
+  }
+}
+
+// Advances the clocks of the clusters that stepped, counts pair work and
+// refreshes the active flags and the count of unfinished clusters.  One
+// block; fixed reduction order.
+__global__ __launch_bounds__(kThreads) void ens_clock(
+    int ncl, const ClusterInfo *__restrict__ cl, double *__restrict__ tc,
+    const double *__restrict__ dt, int *__restrict__ active,
+    long long *__restrict__ nsteps, double *__restrict__ pairs,
+    double t_end, int *__restrict__ count) {
+  __shared__ int red[kThreads];
+  __shared__ double pw[kThreads];
+  int unfinished = 0;
+  double work = 0.0;
+  for (int c = threadIdx.x; c < ncl; c += kThreads) {
+    if (active[c]) {
+      if (dt[c] != 0.0) {
+        tc[c] += dt[c];
+        nsteps[c] += 1;
+        const double n = (double)cl[c].n;
+        work += 4.0 * n * n;  // Tstep + 3 AccJerk evaluations
+      }
+      const int a = (fabs(tc[c]) < t_end) && dt[c] != 0.0;
+      active[c] = a;
+      unfinished += a;
+    }
+  }
+  red[threadIdx.x] = unfinished;
+  pw[threadIdx.x] = work;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      pw[threadIdx.x] += pw[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    count[0] = red[0];
+    pairs[0] += pw[0];
+  }
+}
+
+__global__ void ens_init_flags(int ncl, const double *tc, double t_end,
+                               int *active, int *count, int *err,
+                               double *pairs) {
+  int unfinished = 0;
+  for (int c = 0; c < ncl; ++c) {
+    active[c] = fabs(tc[c]) < t_end;
+    unfinished += active[c];
+  }
+  count[0] = unfinished;
+  err[0] = 0;
+  pairs[0] = 0.0;
+}
+
+inline int blocks_for(long n) {
+  long g = (n + kThreads - 1) / kThreads;
+  return (int)(g < 8192 ? (g > 0 ? g : 1) : 8192);
+}
+
+}  // namespace
+}  // namespace tupan
+
+using namespace tupan;
+
+struct tupan_ensemble {
+  int ncl = 0, ntot = 0;
+  double eta = 0, t_end = 0;
+  std::vector<int> n, off;
+  hipStream_t st = nullptr;
+  // device
+  double *fields[NFIELD] = {};
+  double *slab = nullptr;
+  AccJerkF64Rec *rec = nullptr;
+  ClusterInfo *cl = nullptr;
+  int *cid = nullptr;
+  Item *items = nullptr;
+  int nitems = 0;
+  double *part = nullptr;
+  double *tc = nullptr, *dt = nullptr, *pairs = nullptr;
+  int *active = nullptr, *count = nullptr, *err = nullptr;
+  long long *nsteps = nullptr;
+  // launch shape (autotuner)
+  int ipl = 1, jchunk = 512, waves_per_simd = 8, blocks = 0;
+  bool built = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void ens_release_graph(tupan_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+void ens_free_plan(tupan_ensemble *e) {
+  ens_release_graph(e);
+  if (e->items) (void)hipFree(e->items);
+  if (e->part) (void)hipFree(e->part);
+  if (e->cl) (void)hipFree(e->cl);
+  e->items = nullptr;
+  e->part = nullptr;
+  e->cl = nullptr;
+  e->built = false;
+}
+
+// Work items and partial-sum layout for the current launch shape.
+int ens_plan(tupan_ensemble *e) {
+  ens_free_plan(e);
+  const int P = e->ipl;
+  std::vector<ClusterInfo> cl(e->ncl);
+  std::vector<Item> items;
+  long long base = 0;
+  for (int c = 0; c < e->ncl; ++c) {
+    const int nc = e->n[c];
+    const int nchunk = std::max(1, (nc + e->jchunk - 1) / e->jchunk);
+    cl[c] = {e->off[c], nc, nchunk, base};
+    base += (long long)nchunk * 6 * nc;
+    for (int i0 = 0; i0 < nc; i0 += 64 * P)
+      for (int s = 0; s < nchunk; ++s) {
+        const int j0 = s * e->jchunk;
+        items.push_back({c, i0, j0, std::min(nc, j0 + e->jchunk), s});
+      }
+  }
+  // longest first, ties by cluster then position (deterministic)
+  std::stable_sort(items.begin(), items.end(), [](const Item &a,
+                                                  const Item &b) {
+    return (a.j1 - a.j0) > (b.j1 - b.j0);
+  });
+  e->nitems = (int)items.size();
+  hipError_t er;
+  if ((er = hipMalloc(&e->cl, sizeof(ClusterInfo) * e->ncl)) != hipSuccess ||
+      (er = hipMalloc(&e->items, sizeof(Item) * std::max(1, e->nitems))) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->part, sizeof(double) * std::max(1LL, base))) !=
+          hipSuccess)
+    return hip_fail(er, "ensemble plan allocation");
+  if ((er = hipMemcpy(e->cl, cl.data(), sizeof(ClusterInfo) * e->ncl,
+                      hipMemcpyHostToDevice)) != hipSuccess ||
+      (er = hipMemcpy(e->items, items.data(), sizeof(Item) * e->nitems,
+                      hipMemcpyHostToDevice)) != hipSuccess)
+    return hip_fail(er, "ensemble plan upload");
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                            dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  const long waves = (long)cus * 4 * e->waves_per_simd;
+  e->blocks = (int)std::max(1L, std::min((long)(e->nitems + 3) / 4,
+                                         waves / (kThreads / 64)));
+  e->built = true;
+  return TUPAN_OK;
+}
+
+template <class Pol>
+void launch_pairs(tupan_ensemble *e, typename Pol::Params prm) {
+  if (e->ipl == 2)
+    hipLaunchKernelGGL((ens_pairs<Pol, 2>), dim3(e->blocks), dim3(kThreads),
+                       0, e->st, e->rec, e->items, e->nitems, e->cl,
+                       e->active, e->part, prm);
+  else
+    hipLaunchKernelGGL((ens_pairs<Pol, 1>), dim3(e->blocks), dim3(kThreads),
+                       0, e->st, e->rec, e->items, e->nitems, e->cl,
+                       e->active, e->part, prm);
+}
+
+// The kernels of one ensemble step, in stream order.
+void ens_enqueue_step(tupan_ensemble *e) {
+  Fields f;
+  for (int k = 0; k < NFIELD; ++k) f.f[k] = e->fields[k];
+  const int gb = blocks_for(e->ntot);
+  Outs ts{{e->fields[FTS], e->fields[FTSIJ]}};
+  Outs aj{{e->fields[FAX], e->fields[FAY], e->fields[FAZ], e->fields[FJX],
+           e->fields[FJY], e->fields[FJZ]}};
+  TS::Params tp{e->eta};
+  AJ::Params ap{};
+  hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st, e->ntot,
+                     f, e->cid, e->active, e->rec);
+  launch_pairs<TS>(e, tp);
+  hipLaunchKernelGGL((ens_finish<TS>), dim3(gb), dim3(kThreads), 0, e->st,
+                     e->ntot, e->cid, e->cl, e->active, e->part, ts, tp);
+  hipLaunchKernelGGL(ens_choose_dt, dim3(e->ncl), dim3(kThreads), 0, e->st,
+                     e->cl, e->fields[FTS], e->tc, e->active, e->dt, e->eta,
+                     e->t_end, e->err);
+  for (int pass = 0; pass < 3; ++pass) {
+    if (pass > 0)
+      hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st,
+                         e->ntot, f, e->cid, e->active, e->rec);
+    launch_pairs<AJ>(e, ap);
+    hipLaunchKernelGGL((ens_finish<AJ>), dim3(gb), dim3(kThreads), 0, e->st,
+                       e->ntot, e->cid, e->cl, e->active, e->part, aj, ap);
+    if (pass == 0)
+      hipLaunchKernelGGL(ens_predict, dim3(gb), dim3(kThreads), 0, e->st,
+                         e->ntot, f, e->cid, e->active, e->dt);
+    else
+      hipLaunchKernelGGL(ens_correct, dim3(gb), dim3(kThreads), 0, e->st,
+                         e->ntot, f, e->cid, e->active, e->dt);
+  }
+  hipLaunchKernelGGL(ens_clock, dim3(1), dim3(kThreads), 0, e->st, e->ncl,
+                     e->cl, e->tc, e->dt, e->active, e->nsteps, e->pairs,
+                     e->t_end, e->count);
+}
+
+int ens_capture(tupan_ensemble *e) {
+  ens_release_graph(e);
+  hipError_t er = hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal);
+  if (er != hipSuccess) return hip_fail(er, "hipStreamBeginCapture");
+  ens_enqueue_step(e);
+  er = hipStreamEndCapture(e->st, &e->graph);
+  if (er != hipSuccess) return hip_fail(er, "hipStreamEndCapture");
+  er = hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0);
+  if (er != hipSuccess) return hip_fail(er, "hipGraphInstantiate");
+  return TUPAN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tupan_ensemble_create(tupan_ensemble **out, int ncl, const int *sizes,
+                          double eta, double t_end) {
+  if (!out || !sizes || ncl <= 0)
+    return set_error(TUPAN_ERR_ARG, "ensemble: bad arguments");
+  if (!(eta != 0.0) || !(t_end > 0.0))
+    return set_error(TUPAN_ERR_ARG, "ensemble: eta must be non-zero and "
+                                    "t_end positive");
+  auto *e = new tupan_ensemble();
+  e->ncl = ncl;
+  e->eta = eta;
+  e->t_end = t_end;
+  e->n.assign(sizes, sizes + ncl);
+  e->off.resize(ncl);
+  long long tot = 0;
+  for (int c = 0; c < ncl; ++c) {
+    if (sizes[c] <= 0 || sizes[c] > (1 << 24)) {
+      delete e;
+      return set_error(TUPAN_ERR_ARG, "ensemble: cluster %d has size %d", c,
+                       sizes[c]);
+    }
+    e->off[c] = (int)tot;
+    tot += sizes[c];
+  }
+  if (tot > (1LL << 30)) {
+    delete e;
+    return set_error(TUPAN_ERR_ARG, "ensemble: too many particles");
+  }
+  e->ntot = (int)tot;
+  hipError_t er = hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking);
+  if (er != hipSuccess) {
+    delete e;
+    return hip_fail(er, "hipStreamCreate");
+  }
+  std::vector<int> cid(e->ntot);
+  for (int c = 0; c < ncl; ++c)
+    std::fill(cid.begin() + e->off[c], cid.begin() + e->off[c] + e->n[c], c);
+  const size_t nt = (size_t)e->ntot;
+  if ((er = hipMalloc(&e->slab, sizeof(double) * nt * NFIELD)) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->rec, sizeof(AccJerkF64Rec) * nt)) != hipSuccess ||
+      (er = hipMalloc(&e->cid, sizeof(int) * nt)) != hipSuccess ||
+      (er = hipMalloc(&e->tc, sizeof(double) * ncl)) != hipSuccess ||
+      (er = hipMalloc(&e->dt, sizeof(double) * ncl)) != hipSuccess ||
+      (er = hipMalloc(&e->active, sizeof(int) * ncl)) != hipSuccess ||
+      (er = hipMalloc(&e->nsteps, sizeof(long long) * ncl)) != hipSuccess ||
+      (er = hipMalloc(&e->pairs, sizeof(double))) != hipSuccess ||
+      (er = hipMalloc(&e->count, sizeof(int))) != hipSuccess ||
+      (er = hipMalloc(&e->err, sizeof(int))) != hipSuccess) {
+    int rc = hip_fail(er, "ensemble allocation");
+    tupan_ensemble_destroy(e);
+    return rc;
+  }
+  for (int k = 0; k < NFIELD; ++k) e->fields[k] = e->slab + (size_t)k * nt;
+  (void)hipMemsetAsync(e->slab, 0, sizeof(double) * nt * NFIELD, e->st);
+  (void)hipMemsetAsync(e->tc, 0, sizeof(double) * ncl, e->st);
+  (void)hipMemsetAsync(e->dt, 0, sizeof(double) * ncl, e->st);
+  (void)hipMemsetAsync(e->nsteps, 0, sizeof(long long) * ncl, e->st);
+  er = hipMemcpyAsync(e->cid, cid.data(), sizeof(int) * nt,
+                      hipMemcpyHostToDevice, e->st);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->st);
+  if (er != hipSuccess) {
+    int rc = hip_fail(er, "ensemble init");
+    tupan_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return TUPAN_OK;
+}
+
+int tupan_ensemble_destroy(tupan_ensemble *e) {
+  if (!e) return TUPAN_OK;
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  ens_free_plan(e);
+  void *ptrs[] = {e->slab, e->rec, e->cid,   e->tc,  e->dt,
+                  e->active, e->nsteps, e->pairs, e->count, e->err};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+  return TUPAN_OK;
+}
+
+int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
+                              int waves_per_simd) {
+  if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
+  if ((ipl != 1 && ipl != 2) || jchunk < 16 || waves_per_simd < 1 ||
+      waves_per_simd > 32)
+    return set_error(TUPAN_ERR_ARG, "ensemble launch: ipl 1|2, jchunk >= 16,"
+                                    " 1 <= waves_per_simd <= 32");
+  (void)hipStreamSynchronize(e->st);
+  e->ipl = ipl;
+  e->jchunk = jchunk;
+  e->waves_per_simd = waves_per_simd;
+  ens_free_plan(e);
+  return TUPAN_OK;
+}
+
+// fields: m, rx, ry, rz, e2, vx, vy, vz (host, ntot each); t0: per-cluster
+// start times (may be NULL = 0).  Resets the step counters.
+int tupan_ensemble_set_state(tupan_ensemble *e, const double *const *fields,
+                             const double *t0) {
+  if (!e || !fields) return set_error(TUPAN_ERR_ARG, "NULL argument");
+  const size_t nt = (size_t)e->ntot;
+  for (int k = 0; k < 8; ++k) {
+    if (!fields[k]) return set_error(TUPAN_ERR_ARG, "NULL field %d", k);
+    hipError_t er = hipMemcpyAsync(e->fields[k], fields[k],
+                                   sizeof(double) * nt,
+                                   hipMemcpyHostToDevice, e->st);
+    if (er != hipSuccess) return hip_fail(er, "ensemble upload");
+  }
+  std::vector<double> t(e->ncl, 0.0);
+  if (t0) t.assign(t0, t0 + e->ncl);
+  hipError_t er = hipMemcpyAsync(e->tc, t.data(), sizeof(double) * e->ncl,
+                                 hipMemcpyHostToDevice, e->st);
+  if (er == hipSuccess)
+    er = hipMemsetAsync(e->nsteps, 0, sizeof(long long) * e->ncl, e->st);
+  if (er != hipSuccess) return hip_fail(er, "ensemble upload");
+  hipLaunchKernelGGL(ens_init_flags, dim3(1), dim3(1), 0, e->st, e->ncl,
+                     e->tc, e->t_end, e->active, e->count, e->err, e->pairs);
+  er = hipStreamSynchronize(e->st);
+  return er == hipSuccess ? TUPAN_OK : hip_fail(er, "ensemble upload");
+}
+
+// out: rx, ry, rz, vx, vy, vz (host, ntot each); tc, nsteps per cluster (may
+// be NULL).
+int tupan_ensemble_get_state(tupan_ensemble *e, double *const *out,
+                             double *tc, long long *nsteps) {
+  if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
+  const size_t nt = (size_t)e->ntot;
+  hipError_t er = hipSuccess;
+  for (int k = 0; out && k < 6 && er == hipSuccess; ++k)
+    if (out[k])
+      er = hipMemcpyAsync(out[k], e->fields[k < 3 ? FX + k : FVX + (k - 3)],
+                          sizeof(double) * nt, hipMemcpyDeviceToHost, e->st);
+  if (er == hipSuccess && tc)
+    er = hipMemcpyAsync(tc, e->tc, sizeof(double) * e->ncl,
+                        hipMemcpyDeviceToHost, e->st);
+  if (er == hipSuccess && nsteps)
+    er = hipMemcpyAsync(nsteps, e->nsteps, sizeof(long long) * e->ncl,
+                        hipMemcpyDeviceToHost, e->st);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->st);
+  return er == hipSuccess ? TUPAN_OK : hip_fail(er, "ensemble download");
+}
+
+// Replays the captured step `steps` times (asynchronously).  graph = 0 runs
+// the same kernels launched one by one instead (for comparison).
+int tupan_ensemble_step(tupan_ensemble *e, int steps, int graph) {
+  if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
+  int rc;
+  if (!e->built && (rc = ens_plan(e))) return rc;
+  if (graph && !e->exec && (rc = ens_capture(e))) return rc;
+  for (int s = 0; s < steps; ++s) {
+    if (graph) {
+      hipError_t er = hipGraphLaunch(e->exec, e->st);
+      if (er != hipSuccess) return hip_fail(er, "hipGraphLaunch");
+    } else {
+      ens_enqueue_step(e);
+    }
+  }
+  hipError_t er = hipGetLastError();
+  return er == hipSuccess ? TUPAN_OK : hip_fail(er, "ensemble step");
+}
+
+// Waits for the queued steps; returns the number of unfinished clusters,
+// the pair interactions evaluated since set_state, and the error flag
+// (1: a cluster met a zero or non-finite time step and was stopped).
+int tupan_ensemble_poll(tupan_ensemble *e, int *unfinished, double *pairs,
+                        int *error) {
+  if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
+  int c = 0, ef = 0;
+  double p = 0.0;
+  hipError_t er = hipMemcpyAsync(&c, e->count, sizeof(int),
+                                 hipMemcpyDeviceToHost, e->st);
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(&p, e->pairs, sizeof(double), hipMemcpyDeviceToHost,
+                        e->st);
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(&ef, e->err, sizeof(int), hipMemcpyDeviceToHost,
+                        e->st);
+  if (er == hipSuccess) er = hipStreamSynchronize(e->st);
+  if (er != hipSuccess) return hip_fail(er, "ensemble poll");
+  if (unfinished) *unfinished = c;
+  if (pairs) *pairs = p;
+  if (error) *error = ef;
+  return TUPAN_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@
 //                    (pn_kick_ke, pn_kick_lmom, pn_kick_amom) and
 //                    body.py:481-492 (pn_drift_com_r)
 //   apply_delta      sakura.py:31-36, 39-44
+#include "hermite.hpp"
 #include "launch.hpp"
 
 namespace tupan {
@@ -39,11 +40,12 @@ inline int grid_for(size_t n) {
 template <int ORDER>
 __global__ void __launch_bounds__(kThreads)
     hermite_predict_kernel(int n, P18 x, P18 x0, double tau) {
-#pragma clang fp contract(off)
   constexpr int nd = 6 + 3 * (ORDER / 2);  // r, v + derivatives used
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
        i += gridDim.x * kThreads) {
     double s[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) s[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < nd; ++k) {
       s[k] = x.f[k][i];
@@ -51,27 +53,11 @@ __global__ void __launch_bounds__(kThreads)
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double r = s[d], v = s[3 + d], a = s[6 + d];
-      double dr, dv;
-      if (ORDER == 2) {
-        dr = (a * tau / 2 + v) * tau;
-        dv = a * tau;
-      } else if (ORDER == 4) {
-        const double j = s[9 + d];
-        dr = ((j * tau / 3 + a) * tau / 2 + v) * tau;
-        dv = (j * tau / 2 + a) * tau;
-      } else if (ORDER == 6) {
-        const double j = s[9 + d], sn = s[12 + d];
-        dr = (((sn * tau / 4 + j) * tau / 3 + a) * tau / 2 + v) * tau;
-        dv = ((sn * tau / 3 + j) * tau / 2 + a) * tau;
-      } else {
-        const double j = s[9 + d], sn = s[12 + d], c = s[15 + d];
-        dr = ((((c * tau / 5 + sn) * tau / 4 + j) * tau / 3 + a) * tau / 2 +
-              v) * tau;
-        dv = (((c * tau / 4 + sn) * tau / 3 + j) * tau / 2 + a) * tau;
-      }
-      x.f[d][i] = r + dr;
-      x.f[3 + d][i] = v + dv;
+      double r1, v1;
+      hermite_predict_axis<ORDER>(s[d], s[3 + d], s[6 + d], s[9 + d],
+                                  s[12 + d], s[15 + d], tau, r1, v1);
+      x.f[d][i] = r1;
+      x.f[3 + d][i] = v1;
     }
   }
 }
@@ -81,43 +67,22 @@ __global__ void __launch_bounds__(kThreads)
 template <int ORDER>
 __global__ void __launch_bounds__(kThreads)
     hermite_correct_kernel(int n, P18 x, P18 x0, double tau) {
-#pragma clang fp contract(off)
+  constexpr int nd = 6 + 3 * (ORDER / 2);
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
        i += gridDim.x * kThreads) {
+    double d0[18], d1[18];
+#pragma unroll
+    for (int k = 6; k < 18; ++k) {
+      d0[k] = k < nd ? x0.f[k][i] : 0.0;
+      d1[k] = k < nd ? x.f[k][i] : 0.0;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const double r0 = x0.f[d][i], v0 = x0.f[3 + d][i];
-      const double a0 = x0.f[6 + d][i], a1 = x.f[6 + d][i];
-      double v1, r1;
-      if (ORDER == 2) {
-        v1 = (a0 + a1) * tau / 2 + v0;
-        r1 = (v0 + v1) * tau / 2 + r0;
-      } else if (ORDER == 4) {
-        const double j0 = x0.f[9 + d][i], j1 = x.f[9 + d][i];
-        v1 = ((j0 - j1) * tau / 6 + (a0 + a1)) * tau / 2 + v0;
-        r1 = ((a0 - a1) * tau / 6 + (v0 + v1)) * tau / 2 + r0;
-      } else if (ORDER == 6) {
-        const double j0 = x0.f[9 + d][i], j1 = x.f[9 + d][i];
-        const double s0 = x0.f[12 + d][i], s1 = x.f[12 + d][i];
-        v1 = (((s0 + s1) * tau / 12 + (j0 - j1)) * tau / 5 + (a0 + a1)) *
-                 tau / 2 +
-             v0;
-        r1 = (((j0 + j1) * tau / 12 + (a0 - a1)) * tau / 5 + (v0 + v1)) *
-                 tau / 2 +
-             r0;
-      } else {
-        const double j0 = x0.f[9 + d][i], j1 = x.f[9 + d][i];
-        const double s0 = x0.f[12 + d][i], s1 = x.f[12 + d][i];
-        const double c0 = x0.f[15 + d][i], c1 = x.f[15 + d][i];
-        v1 = ((((c0 - c1) * tau / 20 + (s0 + s1)) * tau / 3 +
-               3 * (j0 - j1)) * tau / 14 +
-              (a0 + a1)) * tau / 2 +
-             v0;
-        r1 = ((((s0 - s1) * tau / 20 + (j0 + j1)) * tau / 3 +
-               3 * (a0 - a1)) * tau / 14 +
-              (v0 + v1)) * tau / 2 +
-             r0;
-      }
+      double r1, v1;
+      hermite_correct_axis<ORDER>(x0.f[d][i], x0.f[3 + d][i], d0[6 + d],
+                                  d1[6 + d], d0[9 + d], d1[9 + d],
+                                  d0[12 + d], d1[12 + d], d0[15 + d],
+                                  d1[15 + d], tau, r1, v1);
       x.f[3 + d][i] = v1;
       x.f[d][i] = r1;
     }
