@@ -49,6 +49,11 @@ def parse():
                    help="target CPU work for the cpu_baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--workload", default="accjerk",
+                   choices=["accjerk", "ensemble"],
+                   help="accjerk: the headline single-evaluation metric; "
+                        "ensemble: the 512-cluster ragged ensemble (adaptive "
+                        "Hermite-4) against looping the single-system path")
     p.add_argument("--pmc-target", action="store_true",
                    help="one evaluation per mode plus a 256 MiB device copy "
                         "(the byte-counter calibration kernel), for "
@@ -139,6 +144,86 @@ def cpu_baseline(ps, n, target_s):
                                          else "bit-identical C restatement")}
 
 
+ENS_CANDIDATES = [(1, 256, 8), (1, 512, 8), (1, 1024, 8), (2, 512, 8),
+                  (1, 512, 4), (1, 512, 16)]
+
+
+def bench_ensemble(args, torch):
+    """512 Plummer clusters, seed 2, N_c log-uniform in [64, 4096], equal
+    masses summing to 1 per cluster, eps^2 = 1e-4; every cluster integrated
+    with adaptive Hermite-4 (eta = 0.02, t_end = 1) on its own block steps.
+    A step advances every cluster by one of its own steps: Tstep + 3 AccJerk
+    = 4 N_c^2 pair interactions.  Timed: `steps` ensemble steps after the
+    autotuner and `warmup` steps.  Baseline: the single-system device path
+    (tupan_amd.integrator, one cluster at a time, kernel launches from the
+    host) on every 16th cluster for the same number of steps."""
+    from tupan_amd import ensemble, kernels
+    from tupan_amd import integrator as ig
+    sizes = ensemble.cluster_sizes(512, seed=2)
+    print("[bench] ensemble: %d clusters, %d particles, sum N^2 = %.3e"
+          % (len(sizes), sizes.sum(), float((sizes.astype(float)**2).sum())),
+          file=sys.stderr, flush=True)
+    phi = lambda p: kernels.compute("phi", p, p, mode="fp64")[0]
+    fields = ensemble.make_ensemble(sizes, seed=2, eps2=1e-4, phi_fn=phi)
+    ens = ensemble.Ensemble(sizes, 0.02, 1.0)
+    print("[bench] autotuning", file=sys.stderr, flush=True)
+    best, rates = ens.autotune(fields, trial_steps=4,
+                               candidates=ENS_CANDIDATES, use_cache=False)
+    ens.set_state(fields)
+    ens.step(args.warmup)
+    ens.poll()
+    _, p0, _ = ens.poll()
+    print("[bench] timing %d ensemble steps" % args.steps, file=sys.stderr,
+          flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    left, p1, err = ens.poll()
+    elapsed = time.perf_counter() - t0
+    ens_rate = (p1 - p0) / elapsed
+    ens.close()
+    # baseline: looping the single-system path per cluster
+    sample = list(range(0, len(sizes), 16))
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    pairs_b = 0.0
+    t_b = 0.0
+    print("[bench] per-cluster loop on %d clusters" % len(sample),
+          file=sys.stderr, flush=True)
+    for c in sample:
+        sl = slice(off[c], off[c + 1])
+        ps = ig.System.from_host({f: fields[f][sl] for f in ensemble.BASE})
+        integ = ig.Integrator(0.02, 0.0, ps, "ahermite4")
+        integ.evolve_step(1.0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            integ.evolve_step(1.0)
+        torch.cuda.synchronize()
+        t_b += time.perf_counter() - t
+        pairs_b += 4.0 * float(sizes[c])**2 * args.steps
+    loop_rate = pairs_b / t_b
+    result = {
+        "metric": "ensemble AccJrk+Tstep Ginteractions/s, 512 clusters, "
+                  "adaptive Hermite-4",
+        "value": ens_rate / 1e9, "unit": "Ginteractions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic",
+        "config": {"workload": "ragged ensemble: 512 Plummer clusters seed 2,"
+                               " N_c log-uniform [64, 4096], eps^2=1e-4, "
+                               "ahermite4 eta=0.02",
+                   "particles": int(sizes.sum()),
+                   "launch": list(best)},
+        "autotune": {"%d,%d,%d" % k: v / 1e9 for k, v in rates.items()},
+        "per_cluster_loop": {"value": loop_rate / 1e9,
+                             "unit": "Ginteractions/s",
+                             "clusters": len(sample),
+                             "speedup": ens_rate / loop_rate},
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -151,6 +236,9 @@ def main():
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "ensemble":
+        bench_ensemble(args, torch)
+        return
     n = args.n
     print("[bench] building Plummer N=%d workload" % n, file=sys.stderr, flush=True)
     ps = make_workload(n)

@@ -150,6 +150,22 @@ int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
                       const double *const *jp, double dt, int flag,
                       double *const *op);
 
+/* ---- pair-selected variants for hybrid schemes -------------------------
+ * No counterpart in the reference (which applies Sakura to every pair).
+ * For a splitting whose close pairs (r < rcut) are drifted by the Sakura
+ * two-body Kepler solve and whose other pairs are kicked:
+ *   tupan_acc_far_dev:    Acc (FP64) over the pairs with r >= rcut only;
+ *   tupan_sakura_near_dev: Sakura over the pairs with r < rcut only (the
+ *                          other pairs contribute nothing).
+ * Arguments as tupan_acc_dev / tupan_sakura_dev; rcut >= 0. */
+int tupan_acc_far_dev(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double rcut, double *const *op,
+                      void *stream);
+int tupan_sakura_near_dev(size_t ni, const double *const *ip, size_t nj,
+                          const double *const *jp, double dt, int flag,
+                          double rcut, double *const *op, int *d_status,
+                          void *stream);
+
 /* ---- adjoints of Acc and Phi ------------------------------------------
  * No counterpart in the reference (it has no differentiation).  For a
  * self-gravitating set (i = j = all particles) with a = acc(ps, ps) or

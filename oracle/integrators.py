@@ -207,8 +207,8 @@ def drift(K, ps, tau, pn=False):
             ps["pn_mr" + d] += ps["pn_mv" + d] * tau
 
 
-def kick(K, ps, tau, pn=False):
-    K.acc(ps)
+def kick(K, ps, tau, pn=False, acc=None):
+    (acc or K.acc)(ps)
     if not pn:
         for d in "xyz":
             ps["v" + d] += ps["a" + d] * tau
@@ -296,6 +296,71 @@ def sakura_do_step(K, ps, tau, eta=None, adaptive=False):
     if adaptive:
         tau = get_sakura_tstep(K, ps, eta, tau)
     sakura_step(K, ps, tau)
+    _advance_clock(ps, tau)
+    return tau
+
+
+# ---- hybrid PN + Sakura scheme ---------------------------------------------
+# Not in the reference (its Sakura acts on every pair, its PN runs in SIA):
+# close pairs (r < rcut) are drifted by the Sakura two-body Kepler solve,
+# the other pairs are kicked together with the PN terms (kick_pn's w
+# scheme and flux bookkeeping):
+#   K(tau/2) [far Newton + PN]  D(tau) [free drift + close-pair Kepler]
+#   K(tau/2)
+
+def acc_far(ps, rcut):
+    """Acc over the pairs with r >= rcut (r^2 > 0), plain numpy."""
+    x = np.stack([ps["rx"], ps["ry"], ps["rz"]])
+    r = x[:, :, None] - x[:, None, :]
+    r2 = (r * r).sum(0)
+    s2 = r2 + (ps["eps2"][:, None] + ps["eps2"][None, :])
+    keep = (r2 > 0) & (r2 >= rcut * rcut)
+    y = np.where(keep, 1.0 / np.sqrt(np.where(keep, s2, 1.0)), 0.0)
+    k = ps["mass"][None, :] * y**3
+    a = -(k[None] * r).sum(2)
+    for n, d in enumerate("xyz"):
+        ps["a" + d] = a[n]
+
+
+def sakura_near(K, ps, dt, flag, rcut):
+    """Sakura deltas from the pairs with r^2 < rcut^2 (self pairs included,
+    as in the full kernel), each pair through the C kernel."""
+    n = len(ps["mass"])
+    x = np.stack([ps["rx"], ps["ry"], ps["rz"]])
+    r = x[:, :, None] - x[:, None, :]
+    r2 = (r * r).sum(0)
+    d = np.zeros((6, n))
+    F8 = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
+    for i in range(n):
+        for j in np.nonzero(r2[i] < rcut * rcut)[0]:
+            out = K.o.call("sakura_kernel", {f: ps[f][i:i + 1] for f in F8},
+                           {f: ps[f][j:j + 1] for f in F8}, dt, flag)
+            d[:, i] += np.array([o[0] for o in out])
+    for k, f in enumerate(("drx", "dry", "drz", "dvx", "dvy", "dvz")):
+        ps[f] = d[k]
+
+
+def hybrid_drift(K, ps, tau, rcut, pn=False):
+    for dd in "xyz":
+        ps["r" + dd] += ps["v" + dd] * tau / 2
+    for flag in (-1, 1):
+        sakura_near(K, ps, tau / 2, flag, rcut)
+        for dd in "xyz":
+            ps["r" + dd] += ps["dr" + dd]
+        for dd in "xyz":
+            ps["v" + dd] += ps["dv" + dd]
+    for dd in "xyz":
+        ps["r" + dd] += ps["v" + dd] * tau / 2
+    if pn:
+        for dd in "xyz":
+            ps["pn_mr" + dd] += ps["pn_mv" + dd] * tau
+
+
+def hybrid_step(K, ps, tau, rcut, pn=True):
+    far = lambda p: acc_far(p, rcut)
+    kick(K, ps, 0.5 * tau, pn, acc=far)
+    hybrid_drift(K, ps, tau, rcut, pn)
+    kick(K, ps, 0.5 * tau, pn, acc=far)
     _advance_clock(ps, tau)
     return tau
 

@@ -200,3 +200,42 @@ def test_trajectory_matches_reference_kernels(torch, name):
         pgot = np.stack([ps.field(f).cpu().numpy() for f in oi.PN_FIELDS])
         pscale = np.abs(pref).max(axis=1, keepdims=True) + 1e-300
         assert (np.abs(pgot - pref) / pscale).max() < 1e-10
+
+
+def test_hybrid_pn_sakura_matches_checker(torch):
+    """Compact PN cluster (N=64, masses U[1,30] normalised, half-mass
+    radius 0.1, unsoftened, pn_order 5 at c = 100) with close pairs
+    (r < 0.02) drifted by Kepler: device driver vs the checker's scheme on
+    the C kernels, 16 steps of 1/4096."""
+    from tupan_amd import integrator as ig
+    from tupan_amd.ics import UniformMass, make_plummer, scale_radius
+    from oracle import Oracle
+    K = oi.Kernels(Oracle("float64", "port"))
+    p = make_plummer(64, 0.0, imf=UniformMass(1.0, 30.0), seed=4,
+                     phi_fn=lambda q: K.o.call("phi_kernel", q, q)[0])
+    scale_radius(p, 0.1 / (1.3048 * 3 * np.pi / 16))
+    init = {f: np.array(getattr(p, f)) for f in BASE}
+    rcut, tau, nstep = 0.02, 1.0 / 4096, 16
+    x = np.stack([init["rx"], init["ry"], init["rz"]])
+    r2 = ((x[:, :, None] - x[:, None, :])**2).sum(0)
+    assert ((r2 > 0) & (r2 < rcut**2)).sum() >= 2
+    ref = {f: v.copy() for f, v in init.items()}
+    for f in oi.PN_FIELDS:
+        ref[f] = np.zeros(64)
+    ref["t_curr"] = 0.0
+    ref["tstep"] = np.zeros(64)
+    K.set_clight(5, 100.0)
+    for _ in range(nstep):
+        oi.hybrid_step(K, ref, tau, rcut)
+    ps = ig.System.from_host(init)
+    integ = ig.Integrator(tau, 0.0, ps, "sakura_pn", rcut=rcut, pn_order=5,
+                          clight=100.0)
+    integ.evolve(nstep * tau)
+    assert int(ps.nstep[0]) == nstep
+    got = ps.to_host(STATE + oi.PN_FIELDS)
+    for f in STATE:
+        err = np.abs(got[f] - ref[f]).max() / np.abs(ref[f]).max()
+        assert err < 1e-11, (f, err)
+    for f in oi.PN_FIELDS:
+        scale = np.abs(ref[f]).max() + 1e-300
+        assert np.abs(got[f] - ref[f]).max() < 1e-9 * scale, f

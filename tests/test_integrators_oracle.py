@@ -268,3 +268,50 @@ def test_hermite_predictor_and_corrector_are_exact_on_polynomials(order):
     oi.hermite_correct(ps1, ps0, tau, order)
     for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
         assert np.allclose(ps1[f], want[f], rtol=1e-13, atol=1e-14), (order, f)
+
+
+# ---- hybrid PN + Sakura scheme (close pairs by Kepler, the rest kicked) --
+
+def test_hybrid_pieces_reduce_to_the_reference_kernels(K):
+    ps = pn_cluster(K, n=12, seed=21)
+    full = K.o.call("acc_kernel", ps, ps)
+    p = oi.copy(ps)
+    oi.acc_far(p, 0.0)
+    for k, d in enumerate("xyz"):
+        assert np.allclose(p["a" + d], full[k], rtol=1e-13, atol=1e-15)
+    sak = K.o.call("sakura_kernel", ps, ps, 0.01, 1)
+    oi.sakura_near(K, p, 0.01, 1, np.inf)
+    for k, f in enumerate(("drx", "dry", "drz", "dvx", "dvy", "dvz")):
+        assert np.array_equal(p[f], sak[k])
+    # with rcut = 0 the scheme is the SIA21 kick-drift-kick with PN
+    a, b = oi.copy(ps), oi.copy(ps)
+    oi.hybrid_step(K, a, 1.0 / 64, 0.0, pn=True)
+    oi.sia21_kdk(K, b, 1.0 / 64, pn=True)
+    for f in STATE + oi.PN_FIELDS:
+        assert np.allclose(a[f], b[f], rtol=1e-13, atol=1e-16), f
+
+
+def test_hybrid_two_bodies_inside_rcut_is_exact_kepler(K):
+    ps, r0, v0, mu = kepler_binary(0.7, 0.3, 1.3, 0.6)
+    oi.hybrid_step(K, ps, 0.9, rcut=10.0, pn=False)
+    r, v = rel_orbit(ps)
+    rx, vx = kepler_exact(r0, v0, mu, 0.9)
+    assert np.allclose(r, rx, rtol=0, atol=1e-11)
+    assert np.allclose(v, vx, rtol=0, atol=1e-11)
+
+
+def test_hybrid_pn_bookkeeping_closes(K):
+    ps = pn_cluster(K, n=8, seed=31, c=30.0)
+    rcut = 0.15
+    x = np.stack([ps["rx"], ps["ry"], ps["rz"]])
+    r2 = ((x[:, :, None] - x[:, None, :])**2).sum(0)
+    assert ((r2 > 0) & (r2 < rcut**2)).any()      # some pairs are close
+    P0, L0, R0 = _mom(ps)
+    E0 = oi.energy(K, ps, pn=True)
+    for _ in range(8):
+        oi.hybrid_step(K, ps, 1.0 / 512, rcut)
+    P1, L1, R1 = _mom(ps)
+    assert np.abs(P1 - P0).max() < 1e-13
+    assert np.abs(L1 - L0).max() < 1e-13
+    assert np.abs(R1 - (R0 + P0 * ps["t_curr"])).max() < 1e-13
+    assert abs(oi.energy(K, ps, pn=True) - E0) < 1e-3 * abs(E0)

@@ -104,6 +104,9 @@ def lib():
         L.tupan_kick_pn_half_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
         L.tupan_pn_flux_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
         L.tupan_apply_delta_dev.argtypes = [_SZ, _PP, _PP, _P]
+        L.tupan_acc_far_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _PP, _P]
+        L.tupan_sakura_near_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _I, _D,
+                                            _PP, _P, _P]
         _PI = ctypes.POINTER(ctypes.c_int)
         _PD = ctypes.POINTER(ctypes.c_double)
         L.tupan_ensemble_create.argtypes = [ctypes.POINTER(_P), _I, _PI, _D,
@@ -182,6 +185,11 @@ def _cargs(name, mode, scalars):
         post.append(inv)  # keep alive; stripped before the call
     elif name == "sakura":
         post = [ctypes.c_double(float(scalars[0])), ctypes.c_int(int(scalars[1]))]
+    elif name == "acc_far":
+        post = [ctypes.c_double(float(scalars[0]))]
+    elif name == "sakura_near":
+        post = [ctypes.c_double(float(scalars[0])), ctypes.c_int(int(scalars[1])),
+                ctypes.c_double(float(scalars[2]))]
     return pre, post
 
 
@@ -209,7 +217,7 @@ def call_dev(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=(), stream=0,
     ia, ja, oa = ptr_array(iptrs), ptr_array(jptrs), ptr_array(optrs)
     fn = getattr(L, "tupan_%s_dev" % name)
     args = pre + [_SZ(ni), ia, _SZ(nj), ja] + post + [oa]
-    if name == "sakura":
+    if name in ("sakura", "sakura_near"):
         args.append(_P(status_ptr or 0))
     args.append(_P(stream or 0))
     rc = fn(*args)

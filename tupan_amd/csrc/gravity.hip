@@ -114,6 +114,27 @@ struct AccF64 : SumOut<3> {
   }
 };
 
+// Acc restricted to the pairs with r^2 >= rc2 (the far field of a
+// hybrid scheme whose close pairs are integrated by the Sakura Kepler
+// drift).  FP64; otherwise AccF64.
+struct AccFarF64 : AccF64 {
+  struct Params { double rc2; };
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return PhiF64::load_i(s, i, NoParams{});
+  }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &prm) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = (r2 > 0.0 && r2 >= prm.rc2) ? y : 0.0;
+    double k = r.m * y * (y * y);
+    a.v[0] -= k * rx;
+    a.v[1] -= k * ry;
+    a.v[2] -= k * rz;
+  }
+};
+
 struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
   static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using JRec = Pos32cJ;
@@ -435,6 +456,16 @@ int tupan_acc_host(int mode, size_t ni, const double *const *ip, size_t nj,
                    const double *const *jp, double *const *op) {
   return dispatch_host<AccF64, AccF32c>(TUPAN_K_ACC, mode, 5, 3, ni, ip, nj,
                                         jp, op);
+}
+
+int tupan_acc_far_dev(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double rcut, double *const *op,
+                      void *stream) {
+  if (!(rcut >= 0.0))
+    return set_error(TUPAN_ERR_ARG, "rcut must be >= 0 (got %g)", rcut);
+  AccFarF64::Params prm{rcut * rcut};
+  return run_engine<AccFarF64>(TUPAN_K_ACC, TUPAN_MODE_FP64, ni, ip, nj, jp,
+                               op, prm, as_stream(stream));
 }
 
 int tupan_acc_jerk_dev(int mode, size_t ni, const double *const *ip,

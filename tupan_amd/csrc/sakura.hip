@@ -17,6 +17,7 @@ struct SakuraParams {
   int flag;
   int budget;
   int *status;
+  double rc2;  // only pairs with r^2 < rc2 evolve (infinity: all pairs)
 };
 
 struct SakuraF64 : SumOut<6> {
@@ -39,6 +40,9 @@ struct SakuraF64 : SumOut<6> {
                               const Params &prm) {
     double st[6] = {q.x - r.x,   q.y - r.y,   q.z - r.z,
                     q.vx - r.vx, q.vy - r.vy, q.vz - r.vz};
+    if (prm.rc2 < INFINITY &&
+        !(st[0] * st[0] + st[1] * st[1] + st[2] * st[2] < prm.rc2))
+      return;
     const double r0[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
     double m = q.m + r.m;
     double e2 = q.e2 + r.e2;
@@ -59,7 +63,19 @@ extern "C" {
 int tupan_sakura_dev(size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double dt, int flag,
                      double *const *op, int *d_status, void *stream) {
-  SakuraParams prm{dt, flag, sakura_budget(), d_status};
+  SakuraParams prm{dt, flag, sakura_budget(), d_status, INFINITY};
+  return run_engine<SakuraF64>(TUPAN_K_SAKURA, TUPAN_MODE_FP64, ni, ip, nj,
+                               jp, op, prm,
+                               reinterpret_cast<hipStream_t>(stream));
+}
+
+int tupan_sakura_near_dev(size_t ni, const double *const *ip, size_t nj,
+                          const double *const *jp, double dt, int flag,
+                          double rcut, double *const *op, int *d_status,
+                          void *stream) {
+  if (!(rcut >= 0.0))
+    return set_error(TUPAN_ERR_ARG, "rcut must be >= 0 (got %g)", rcut);
+  SakuraParams prm{dt, flag, sakura_budget(), d_status, rcut * rcut};
   return run_engine<SakuraF64>(TUPAN_K_SAKURA, TUPAN_MODE_FP64, ni, ip, nj,
                                jp, op, prm,
                                reinterpret_cast<hipStream_t>(stream));
@@ -82,7 +98,8 @@ int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
   rc = host_call(ni, ip, nj, jp, 8, op, 6,
                  [&](const double *const *di, const double *const *dj,
                      double *const *dout, hipStream_t st) {
-                   SakuraParams prm{dt, flag, sakura_budget(), status};
+                   SakuraParams prm{dt, flag, sakura_budget(), status,
+                                    INFINITY};
                    return run_engine<SakuraF64>(TUPAN_K_SAKURA,
                                                 TUPAN_MODE_FP64, ni, di, nj,
                                                 dj, dout, prm, st);

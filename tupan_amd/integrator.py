@@ -14,6 +14,8 @@ Provided (the reference's method names):
   SIA      sia21s.kdk/.dkd, sia21a.kdk/.dkd  (tupan/integrator/sia.py; shared
            steps, with PN corrections when pn_order > 0)
   Sakura   sakura, asakura                    (tupan/integrator/sakura.py)
+  SakuraPN sakura_pn: close pairs by the Sakura Kepler drift, the rest and
+           the PN terms by kicks (a hybrid the reference does not have)
 
 Differences from the reference, all deliberate:
   * the state is not copied per step (``ps.copy()``, hermite.py:79): the
@@ -383,14 +385,81 @@ class Sakura(Base):
         return ps
 
 
+class SakuraPN(SIA):
+    """Hybrid scheme for compact PN clusters (no reference counterpart: the
+    reference's Sakura acts on every pair and has no PN terms).  Pairs closer
+    than ``rcut`` are drifted by the Sakura two-body Kepler solve; the other
+    pairs are kicked together with the PN terms, with kick_pn's w scheme and
+    flux bookkeeping (sia.py:142-156):
+
+        K(tau/2) [far Newton + PN]  D(tau) [free drift + close-pair Kepler]
+        K(tau/2)
+
+    Shared fixed steps tau = min(t_end - t, eta)."""
+
+    PROVIDED_METHODS = ["sakura_pn"]
+
+    def __init__(self, eta, time, ps, method, rcut=1e-3, **kw):
+        super(SIA, self).__init__(eta, time, ps, method, **kw)
+        self.rcut = float(rcut)
+        for f in PN_FIELDS:
+            ps.field(f)
+
+    def kick(self, ps, tau):
+        ps._pair("acc_far", self.rcut, mode="fp64")
+        v = ps.ptrs(("vx", "vy", "vz"))
+        a = ps.ptrs(("ax", "ay", "az"))
+        st = _stream(ps)
+        if not ps.include_pn_corrections:
+            _lib.call_update("kick", ps.n, v, a, float(tau), st)
+            return
+        w = ps.ptrs(("wx", "wy", "wz"))
+        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
+        ps.set_pnacc(self.clight)
+        _lib.call_update(
+            "pn_flux", ps.n,
+            ps.ptrs(("mass", "rx", "ry", "rz", "vx", "vy", "vz",
+                     "pnax", "pnay", "pnaz")),
+            ps.ptrs(("wx", "wy", "wz", "pn_ke", "pn_mvx", "pn_mvy", "pn_mvz",
+                     "pn_amx", "pn_amy", "pn_amz")), float(tau), st)
+        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
+
+    def drift(self, ps, tau):
+        r = ps.ptrs(("rx", "ry", "rz"))
+        v = ps.ptrs(("vx", "vy", "vz"))
+        x = r + v
+        d = ps.ptrs(("drx", "dry", "drz", "dvx", "dvy", "dvz"))
+        st = _stream(ps)
+        _lib.call_update("drift", ps.n, r, v, float(tau), 1, None, None, st)
+        for flag in (-1, 1):
+            ps._pair("sakura_near", tau / 2, flag, self.rcut,
+                     status=self.status)
+            _lib.call_update("apply_delta", ps.n, x, d, st)
+        _lib.call_update("drift", ps.n, r, v, float(tau), 1, None, None, st)
+        if ps.include_pn_corrections:
+            # pn_mr += pn_mv * tau with r untouched: a drift of the
+            # accumulators alone
+            _lib.call_update(
+                "drift", ps.n, ps.ptrs(("pn_mrx", "pn_mry", "pn_mrz")),
+                ps.ptrs(("pn_mvx", "pn_mvy", "pn_mvz")), float(tau), 0, None,
+                None, st)
+
+    def do_step(self, ps, tau):
+        self.kick(ps, 0.5 * tau)
+        self.drift(ps, tau)
+        self.kick(ps, 0.5 * tau)
+        self._advance_clock(ps, tau)
+        return ps
+
+
 class Integrator(object):
     """Method-name dispatch (tupan/integrator/__init__.py:101-160)."""
 
     PROVIDED_METHODS = (Hermite.PROVIDED_METHODS + SIA.PROVIDED_METHODS
-                        + Sakura.PROVIDED_METHODS)
+                        + Sakura.PROVIDED_METHODS + SakuraPN.PROVIDED_METHODS)
 
     def __init__(self, eta, time, ps, method=None, **kw):
-        for cls in (Hermite, SIA, Sakura):
+        for cls in (Hermite, SIA, Sakura, SakuraPN):
             if method in cls.PROVIDED_METHODS:
                 self.integrator = cls(eta, time, ps, method, **kw)
                 break

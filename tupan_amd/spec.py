@@ -28,6 +28,12 @@ KERNELS = {
     "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), False, None),
     "sakura": Kernel("sakura", 6, F8,
                      ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
+    # pair-selected variants for hybrid schemes (no reference counterpart):
+    # Acc over the pairs with r >= rcut, Sakura over the pairs with r < rcut
+    "acc_far": Kernel("acc_far", 1, F5, ("ax", "ay", "az"), False, 20),
+    "sakura_near": Kernel("sakura_near", 6, F8,
+                          ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False,
+                          None),
     # reverse-mode derivatives (no reference counterpart): inputs F5 plus the
     # upstream gradient dL/da (gx, gy, gz) or dL/dphi (h); outputs dL/dr and
     # dL/dm.  Flops per pair count every IEEE operation of the FP64 pair
