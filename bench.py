@@ -50,10 +50,14 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--workload", default="accjerk",
-                   choices=["accjerk", "ensemble"],
+                   choices=["accjerk", "sizes", "ensemble", "pn"],
                    help="accjerk: the headline single-evaluation metric; "
+                        "sizes: Acc, AccJerk, SnapCrackle at N = 16K, 64K, "
+                        "256K in both modes with errors vs FP64; "
                         "ensemble: the 512-cluster ragged ensemble (adaptive "
-                        "Hermite-4) against looping the single-system path")
+                        "Hermite-4) against looping the single-system path; "
+                        "pn: the compact post-Newtonian cluster with Sakura "
+                        "close pairs")
     p.add_argument("--pmc-target", action="store_true",
                    help="one evaluation per mode plus a 256 MiB device copy "
                         "(the byte-counter calibration kernel), for "
@@ -224,6 +228,153 @@ def bench_ensemble(args, torch):
     print(json.dumps(result))
 
 
+def bench_sizes(args, torch):
+    """The single-evaluation kernels at N in {16384, 65536, 262144}: Plummer
+    seed 1, Salpeter masses [0.1, 100], eps = 4/N.  Acc, AccJerk and
+    SnapCrackle (its a, j inputs from the FP64 AccJerk) in FP64 and FP32C:
+    interactions/s from the live engine timing, and per-particle relative
+    errors |x - x64| / |x64| (max, RMS) of every output vector against the
+    FP64 kernel."""
+    from tupan_amd import device, spec
+    rows = []
+    for n in (16384, 65536, 262144):
+        print("[bench] sizes: N=%d" % n, file=sys.stderr, flush=True)
+        ps = make_workload(n)
+        d = device.to_device(ps, spec.F8)
+        aj = device.compute("acc_jerk", d, d, mode="fp64")
+        for k, f in enumerate(("ax", "ay", "az", "jx", "jy", "jz")):
+            d[f] = aj[k].clone()
+        for name in ("acc", "acc_jerk", "snap_crackle"):
+            res = {}
+            for mode in ("fp64", "fp32c"):
+                out = tuple(torch.empty(n, dtype=torch.float64, device="cuda")
+                            for _ in spec.KERNELS[name].outs)
+                step = lambda: device.compute(name, d, d, mode=mode, out=out)
+                reps = max(2, min(20, int(2e11 / (n * n))))
+                _, k_ms, _ = time_steps(step, reps, 1, torch)
+                res[mode] = (out, k_ms)
+            pairs = float(n) * n
+            row = {"n": n, "kernel": name}
+            for mode in ("fp64", "fp32c"):
+                row[mode + "_Gint_s"] = pairs / (res[mode][1] * 1e-3) / 1e9
+                row[mode + "_kernel_ms"] = res[mode][1]
+            got, ref = res["fp32c"][0], res["fp64"][0]
+            for g in range(len(got) // 3):
+                G = torch.stack(got[3 * g:3 * g + 3])
+                R = torch.stack(ref[3 * g:3 * g + 3])
+                e = (G - R).norm(dim=0) / R.norm(dim=0)
+                tag = ("acc", "jerk", "snap", "crackle")[
+                    g + (2 if name == "snap_crackle" else 0)]
+                row[tag + "_max"] = float(e.max())
+                row[tag + "_rms"] = float(e.pow(2).mean().sqrt())
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+    big = [r for r in rows if r["n"] == 65536 and r["kernel"] == "acc_jerk"][0]
+    result = {
+        "metric": "single-evaluation kernels at N=16K/64K/256K, FP32C and "
+                  "FP64, Ginteractions/s and errors vs FP64",
+        "value": big["fp32c_Gint_s"], "unit": "Ginteractions/s",
+        "n_gpus": 1, "steps": None, "warmup": 1, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "Plummer seed 1, Salpeter [0.1,100], eps=4/N;"
+                               " Acc, AccJerk, SnapCrackle"},
+        "rows": rows,
+    }
+    print(json.dumps(result))
+
+
+def bench_pn(args, torch):
+    """Compact PN cluster: N=2048, seed 4, masses uniform in [1, 30]
+    normalised to 1, Plummer scaled to half-mass radius 0.1, unsoftened,
+    pn_order 5 (1PN + 2PN + 2.5PN) at c = 1e3; pairs closer than 1e-3 are
+    drifted by the FP64 Sakura Kepler solve, the rest and the PN terms are
+    kicked (integrator 'sakura_pn', dt = 1/8192) for 0.1 time units.  A
+    step evaluates far Acc, PNAcc and near Sakura twice each: 6 N^2 pair
+    visits.  Reported: energy error including the PN work accumulator,
+    closure of momentum and angular momentum with their PN accumulators, and
+    pair throughput against the reference's C kernels for the same pair
+    visits (Acc and PNAcc, and Sakura over every pair as the reference
+    applies it) on a bounded sample of host steps."""
+    from tupan_amd import integrator as ig
+    from tupan_amd import kernels
+    from tupan_amd.ics import UniformMass, make_plummer, scale_radius
+    n, dt, t_end, rcut, c = 2048, 1.0 / 8192, 0.1, 1e-3, 1e3
+    phi = lambda p: kernels.compute("phi", p, p, mode="fp64")[0]
+    p = make_plummer(n, 0.0, imf=UniformMass(1.0, 30.0), seed=4, phi_fn=phi)
+    scale_radius(p, 0.1 / (1.3048 * 3 * np.pi / 16))
+    ps = ig.System.from_host(p)
+    integ = ig.Integrator(dt, 0.0, ps, "sakura_pn", rcut=rcut, pn_order=5,
+                          clight=c)
+
+    def invariants(s):
+        m = s.mass
+        r = torch.stack([s.rx, s.ry, s.rz])
+        v = torch.stack([s.vx, s.vy, s.vz])
+        P = (m * v).sum(1) + torch.stack([s.field("pn_mv" + d).sum()
+                                          for d in "xyz"])
+        L = (m * torch.linalg.cross(r, v, dim=0)).sum(1) + torch.stack(
+            [s.field("pn_am" + d).sum() for d in "xyz"])
+        return s.energy(), P.cpu().numpy(), L.cpu().numpy()
+
+    E0, P0, L0 = invariants(ps)
+    x = np.stack([p.rx, p.ry, p.rz])
+    close0 = 0
+    for i in range(0, n, 256):
+        d2 = ((x[:, i:i + 256, None] - x[:, None, :])**2).sum(0)
+        close0 += int(((d2 > 0) & (d2 < rcut * rcut)).sum())
+    print("[bench] pn: N=%d, %d close pairs at t=0, %d steps"
+          % (n, close0 // 2, int(round(t_end / dt))), file=sys.stderr,
+          flush=True)
+    integ.evolve_step(t_end)                      # warm-up step
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nstep = 0
+    while abs(ps.t_curr) < t_end:
+        integ.evolve_step(t_end)
+        nstep += 1
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    integ.integrator.check_status()
+    E1, P1, L1 = invariants(ps)
+    visits = 6.0 * n * n * nstep
+    gpu_rate = visits / elapsed
+    # the reference's C kernels for the same pair visits, on the host
+    import oracle
+    kind = "reference" if oracle.available("float64", "reference") else "port"
+    o = oracle.Oracle("float64", kind)
+    h = ps.to_host()
+    inv = tuple(c**-k for k in range(1, 8))
+    t = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t < args.cpu_seconds / 2 or reps == 0:
+        o.call("acc_kernel", h, h)
+        o.call("pnacc_kernel", h, h, 5, *inv)
+        o.call("sakura_kernel", h, h, dt / 2, 1)
+        reps += 1
+    cpu_rate = 3.0 * n * n * reps / (time.perf_counter() - t)
+    result = {
+        "metric": "PN cluster pair visits/s (far Acc + PNAcc + near Sakura),"
+                  " N=2048",
+        "value": gpu_rate / 1e9, "unit": "Gpairs/s", "n_gpus": 1,
+        "steps": nstep, "warmup": 1, "ms_per_step": 1e3 * elapsed / nstep,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": "compact PN cluster N=2048 seed 4, m~U[1,30],"
+                               " r_h=0.1, eps=0, pn_order 5, c=1e3, Sakura "
+                               "pairs r<1e-3, dt=1/8192, t=0.1",
+                   "close_pairs_t0": close0 // 2},
+        "energy_rel_err": abs(E1 - E0) / abs(E0),
+        "momentum_closure": float(np.abs(P1 - P0).max()),
+        "angular_momentum_closure": float(np.abs(L1 - L0).max()),
+        "cpu_baseline": {"value": cpu_rate / 1e9, "unit": "Gpairs/s",
+                         "cores": 1, "kind": kind,
+                         "sample": "%d x (Acc + PNAcc order 5 + Sakura over "
+                                   "all pairs), N=2048, 1 thread" % reps},
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -238,6 +389,12 @@ def main():
         sys.exit(1)
     if args.workload == "ensemble":
         bench_ensemble(args, torch)
+        return
+    if args.workload == "pn":
+        bench_pn(args, torch)
+        return
+    if args.workload == "sizes":
+        bench_sizes(args, torch)
         return
     n = args.n
     print("[bench] building Plummer N=%d workload" % n, file=sys.stderr, flush=True)
