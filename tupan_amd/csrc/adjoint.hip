@@ -71,10 +71,11 @@ struct AccAdjF32c : F32cBase<AccAdjF32c, AccAdjF64, 4> {
     float m, e2, gx, gy, gz;
     int i;
   };
-  __device__ static JRec pack(const Soa &s, int j, double sc) {
-    return {s.f[RX][j],         s.f[RY][j],        s.f[RZ][j],
-            (float)s.f[M][j],   (float)s.f[E2][j], near2_of(s.f[M][j], sc),
-            (float)s.f[GX][j],  (float)s.f[GY][j], (float)s.f[GZ][j]};
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
+    return {s.f[RX][j],        s.f[RY][j],        s.f[RZ][j],
+            (float)s.f[M][j],  (float)s.f[E2][j],
+            near_threshold(s.f[M][j], s.f[E2][j], c),
+            (float)s.f[GX][j], (float)s.f[GY][j], (float)s.f[GZ][j]};
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return {s.f[RX][i],        s.f[RY][i],        s.f[RZ][i],
@@ -82,8 +83,8 @@ struct AccAdjF32c : F32cBase<AccAdjF32c, AccAdjF64, 4> {
             (float)s.f[GY][i], (float)s.f[GZ][i], i};
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
-                                float rz, float r2, bool far, A32 &a) {
-    float y = rsq32(r2 + (q.e2 + r.e2));
+                                float rz, float s2, bool far, A32 &a) {
+    float y = rsq32(s2);
     y = far ? y : 0.0f;
     float y2 = y * y;
     float y3 = y * y2;
@@ -139,18 +140,18 @@ struct PhiAdjF32c : F32cBase<PhiAdjF32c, PhiAdjF64, 4> {
     float m, e2, h;
     int i;
   };
-  __device__ static JRec pack(const Soa &s, int j, double sc) {
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
     return {s.f[RX][j],       s.f[RY][j],        s.f[RZ][j],
-            (float)s.f[M][j], (float)s.f[E2][j], near2_of(s.f[M][j], sc),
-            (float)s.f[H][j]};
+            (float)s.f[M][j], (float)s.f[E2][j],
+            near_threshold(s.f[M][j], s.f[E2][j], c), (float)s.f[H][j]};
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return {s.f[RX][i],       s.f[RY][i],        s.f[RZ][i],
             (float)s.f[M][i], (float)s.f[E2][i], (float)s.f[H][i], i};
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
-                                float rz, float r2, bool far, A32 &a) {
-    float y = rsq32(r2 + (q.e2 + r.e2));
+                                float rz, float s2, bool far, A32 &a) {
+    float y = rsq32(s2);
     y = far ? y : 0.0f;
     float w = (q.h * r.m + r.h * q.m) * (y * y * y);
     a.v[0] += w * rx;

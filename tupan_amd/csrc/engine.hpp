@@ -38,6 +38,13 @@ struct Soa {
 struct Out {
   double *f[6];
 };
+// Per-call values the FP32C packers need (see ctx_final_kernel and
+// tile_origin_kernel).
+struct PackCtx {
+  double factor;          // near-field factor: near2_j = m_j * factor
+  double e2max;           // largest i-side half softening
+  const double *origins;  // per-tile FP64 origins (4 doubles per tile)
+};
 
 constexpr int kWaves = 4;       // waves per block
 constexpr int kBlock = 64 * kWaves;
@@ -91,7 +98,7 @@ template <class Pol, int P>
 __global__ __launch_bounds__(kBlock) void pair_engine(
     Soa is, Soa js, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
     int jchunk, double *__restrict__ part, Out out,
-    typename Pol::Params prm) {
+    typename Pol::Params prm, const double *__restrict__ aux) {
   const int lane = threadIdx.x & 63;
   const int tile = blockIdx.x * kWaves + (threadIdx.x >> 6);
   const int i0 = tile * 64 * P;
@@ -117,6 +124,8 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
     // met j0 + k in its near field, and after the tile those (k, p) are
     // re-evaluated in FP64 for exactly those lanes (near_pair).  The hot loop carries no FP64
     // code, so the register budget stays that of the FP32 path.
+    // j0 is a multiple of kTile (jchunk is), so tile j0 / kTile is the one
+    // the packer's per-tile origins refer to.
     for (int j0 = jb; j0 < je; j0 += kTile) {
       const int nk = min(kTile, je - j0);
       typename Pol::A32 a[P];
@@ -125,6 +134,7 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
       for (int p = 0; p < P; ++p) {
         Pol::zero(a[p]);
         near[p] = 0u;
+        Pol::tile_begin(ist[p], aux, j0 / kTile);
       }
       // the record of j+1 is fetched (s_load) while j is computed
       typename Pol::JRec r = jr[j0];
@@ -207,38 +217,90 @@ __global__ __launch_bounds__(256) void finish_kernel(
   Pol::finish(v, out, i, prm);
 }
 
-// Packs the caller's SoA j fields into the policy's record.  `scale` (may
-// be null) is the near-field factor written by near_scale_kernel.
-template <class Pol>
-__global__ __launch_bounds__(256) void pack_kernel(
-    Soa js, int nj, typename Pol::JRec *__restrict__ jr,
-    const double *__restrict__ scale) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nj) jr[j] = Pol::pack(js, j, scale ? *scale : 0.0);
-}
+// Per-call context of the FP32C packers, in device memory:
+//   ctx[0] near-field factor, ctx[1] max i-side e2, ctx[2..] per-tile
+//   origins (4 doubles per tile of kTile j's).
+// Near field: a pair (i, j) is "near" when r_ij^2 < (m_j / <m>) *
+// (c * nj^(-1/3))^2 -- inside c mean interparticle spacings of a unit-size
+// system, widened by the square root of the j-mass in units of the mean
+// mass, i.e. a fixed threshold on the pair force m_j / r^2 -- so the
+// factor is c^2 nj^(1/3) / sum_j m_j (0 when c = 0: no near field beyond
+// the r^2 = 0 pairs).
+constexpr int kCtxBlocks = 64;
 
-// Near-field radius for the FP32C policies.  A pair (i, j) is "near" when
-// r_ij^2 < (m_j / <m>) * (c * nj^(-1/3))^2: inside c mean interparticle
-// spacings of a unit-size system, widened by the square root of the
-// j-mass in units of the mean mass, i.e. a fixed threshold on the pair
-// force m_j / r^2.  Writes the factor (c^2 nj^(1/3)) / sum_j m_j so that
-// near2_j = m_j * factor.  One block, fixed reduction order.
-static __global__ __launch_bounds__(1024) void near_scale_kernel(
-    const double *__restrict__ m, int nj, double c, double *__restrict__ out) {
-  __shared__ double part[1024];
-  double s = 0.0;
-  for (int j = threadIdx.x; j < nj; j += 1024) s += m[j];
-  part[threadIdx.x] = s;
+// Partial sums of m_j and maxima of e2_i, one pair per block.
+static __global__ __launch_bounds__(256) void ctx_partial_kernel(
+    const double *__restrict__ m, int nj, const double *__restrict__ e2,
+    int ni, double *__restrict__ part) {
+  __shared__ double ps[256], px[256];
+  double s = 0.0, x = 0.0;
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < nj; j += kCtxBlocks * 256)
+    s += m[j];
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < ni; i += kCtxBlocks * 256)
+    x = fmax(x, e2[i]);
+  ps[threadIdx.x] = s;
+  px[threadIdx.x] = x;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      ps[threadIdx.x] += ps[threadIdx.x + w];
+      px[threadIdx.x] = fmax(px[threadIdx.x], px[threadIdx.x + w]);
+    }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double msum = part[0];
-    double f = c * c * cbrt((double)nj) / msum;
-    out[0] = (msum > 0.0 && f < 1e300) ? f : 0.0;
+    part[2 * blockIdx.x] = ps[0];
+    part[2 * blockIdx.x + 1] = px[0];
   }
+}
+
+// Combines the block partials in block order (run-to-run reproducible).
+static __global__ void ctx_final_kernel(const double *__restrict__ part,
+                                        int nj, double c,
+                                        double *__restrict__ ctx) {
+  if (threadIdx.x != 0) return;
+  double msum = 0.0, e2max = 0.0;
+  for (int b = 0; b < kCtxBlocks; ++b) {
+    msum += part[2 * b];
+    e2max = fmax(e2max, part[2 * b + 1]);
+  }
+  const double f = c * c * cbrt((double)nj) / msum;
+  ctx[0] = (c > 0.0 && msum > 0.0 && f < 1e300) ? f : 0.0;
+  ctx[1] = e2max;
+}
+
+// Per-tile origin of the fields `f[0..2]` (the midpoint of their range over
+// the tile's j's), for policies with kVelOrigins.  One thread per tile.
+static __global__ __launch_bounds__(256) void tile_origin_kernel(
+    const double *__restrict__ f0, const double *__restrict__ f1,
+    const double *__restrict__ f2, int nj, double *__restrict__ origins) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t * kTile >= nj) return;
+  const double *f[3] = {f0, f1, f2};
+  const int je = min(nj, (t + 1) * kTile);
+  for (int k = 0; k < 3; ++k) {
+    double lo = f[k][t * kTile], hi = lo;
+    for (int j = t * kTile + 1; j < je; ++j) {
+      lo = fmin(lo, f[k][j]);
+      hi = fmax(hi, f[k][j]);
+    }
+    origins[4 * t + k] = 0.5 * (lo + hi);
+  }
+  origins[4 * t + 3] = 0.0;
+}
+
+// Packs the caller's SoA j fields into the policy's record; FP32C policies
+// read the context written by the kernels above.
+template <class Pol>
+__global__ __launch_bounds__(256) void pack_kernel(
+    Soa js, int nj, typename Pol::JRec *__restrict__ jr,
+    const double *__restrict__ ctx) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nj) return;
+  if constexpr (Pol::kTwoLevel)
+    jr[j] = Pol::pack(js, j, PackCtx{ctx[0], ctx[1], ctx + 2});
+  else
+    jr[j] = Pol::pack(js, j, 0.0);
 }
 
 // Shared defaults for policies whose outputs are plain sums.
@@ -258,6 +320,7 @@ struct SumOut {
 };
 
 struct NoParams {};
+
 
 template <int N>
 struct F64Acc {

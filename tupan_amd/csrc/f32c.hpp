@@ -1,22 +1,38 @@
 // f32c.hpp -- common shape of the FP32C ("FP32-compensated") policies.
 //
-//   * pair separations (and velocity differences) are formed in FP64 and
-//     rounded once to FP32 (dsub32);
+//   * pair separations are formed in FP64 and rounded once to FP32
+//     (dsub32); velocity differences are formed from FP32 offsets to a
+//     per-tile FP64 origin (policies with kVelOrigins): the i-side offset is
+//     rounded once per tile, so its rounding is shared by 32 pairs only and
+//     averages out over the tiles instead of adding up over all j;
 //   * the rest of the pair arithmetic is FP32 with v_rsq_f32;
 //   * sums run in FP32 over a tile of kTile j's and are folded into FP64
 //     accumulators after every tile;
-//   * near field: a pair with r^2 < near2_j contributes nothing in FP32 and
-//     is re-evaluated by the FP64 policy from the caller's FP64 arrays.
+//   * near field: a pair whose softened s^2 is below the j record's
+//     threshold contributes nothing in FP32 and is re-evaluated by the FP64
+//     policy from the caller's FP64 arrays.
 #pragma once
 #include "engine.hpp"
 
 namespace tupan {
 
+// Threshold on the FP32 s^2 = r^2 + e2_i + e2_j below which a pair is near:
+// (near2_j + e2_j + e2max) widened by 2^-20, so that every pair with r^2 = 0
+// (s^2 = e2_i + e2_j rounded) is near whatever the rounding, and a pair
+// with r^2 < near2_j always is.  near2_j = max(m_j * factor, FLT_MIN).
+__device__ __forceinline__ float near_threshold(double m, double e2j,
+                                                const PackCtx &c) {
+  const double n2 = fmax(m * c.factor, 1.1754943508222875e-38);
+  return (float)((n2 + e2j + c.e2max) * (1.0 + 0x1p-20));
+}
+
 // Common shape of the FP32C policies: P64 is the FP64 policy used for the
-// near field; the derived policy supplies JRec, I, pack, load_i and body32.
+// near field; the derived policy supplies JRec (with a float `near2`
+// threshold), I (with float `e2` and int `i`), pack, load_i and body32.
 template <class Derived, class P64, int N>
 struct F32cBase : SumOut<N> {
   static constexpr bool kTwoLevel = true;
+  static constexpr int kVelOrigins = 0;
   using Params = NoParams;
   using A64 = F64Acc<N>;
   using A32 = F32Acc<N>;
@@ -24,43 +40,41 @@ struct F32cBase : SumOut<N> {
   __device__ static void zero(A32 &a) { zero_acc(a); }
   __device__ static void fold(const A32 &a, A64 &b) { fold_acc(a, b); }
   __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  template <class I>
+  __device__ static void tile_begin(I &, const double *, int) {}
 
-  // Separation and the near-field test.  r^2 uses explicit FMAs so that
-  // pair32 and near_pair take bit-identical decisions.  near2 >= FLT_MIN,
-  // so coincident particles (r^2 = 0, masked by the reference) always take
-  // the FP64 path, which applies the mask (as does a NaN separation).
+  // Separation, softened s^2 and the near-field test.  s^2 uses explicit
+  // FMAs so that pair32 and near_pair take bit-identical decisions; a NaN
+  // separation is near too (the FP64 path propagates it).
   template <class I, class JRec>
   __device__ static bool near_test(const I &q, const JRec &r, float &rx,
-                                   float &ry, float &rz, float &r2) {
+                                   float &ry, float &rz, float &s2) {
     rx = dsub32(q.x, r.x);
     ry = dsub32(q.y, r.y);
     rz = dsub32(q.z, r.z);
-    r2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-    return !(r2 >= r.near2);
+    s2 = __builtin_fmaf(
+        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+    return !(s2 >= r.near2);
   }
   // FP32 contribution of a far pair; near pairs contribute zero here.
   // Returns the wave's mask of far lanes.
   template <class I, class JRec>
   __device__ static uint64_t pair32(const I &q, const JRec &r, A32 &a) {
-    float rx, ry, rz, r2;
-    const bool far = !near_test(q, r, rx, ry, rz, r2);
-    Derived::body32(q, r, rx, ry, rz, r2, far, a);
+    float rx, ry, rz, s2;
+    const bool far = !near_test(q, r, rx, ry, rz, s2);
+    Derived::body32(q, r, rx, ry, rz, s2, far, a);
     return __builtin_amdgcn_ballot_w64(far);
   }
   // FP64 contribution of a near pair, from the caller's FP64 arrays.
   template <class I, class JRec>
   __device__ static void near_pair(const I &q, const JRec &r, A64 &a64,
                                    const Soa &is, const Soa &js, int j) {
-    float rx, ry, rz, r2;
-    if (near_test(q, r, rx, ry, rz, r2)) {
+    float rx, ry, rz, s2;
+    if (near_test(q, r, rx, ry, rz, s2)) {
       typename P64::Params p64{};
       P64::pair(P64::load_i(is, q.i, p64), P64::pack(js, j, 0.0), a64, p64);
     }
   }
 };
-
-__device__ __forceinline__ float near2_of(double m, double sc) {
-  return fmaxf((float)(m * sc), 1.17549435e-38f);  // >= FLT_MIN
-}
 
 }  // namespace tupan

@@ -1,18 +1,19 @@
 // gravity.hip -- Phi, Acc, AccJerk, SnapCrackle and Tstep on the pair engine.
 //
 // Each kernel has an FP64 policy (FP64 arithmetic, hardware rsq refined by
-// one Newton-Householder step) and, except Tstep, an FP32C policy:
+// one Newton-Householder step) and, except Tstep, an FP32C policy
+// (f32c.hpp):
 //   * r_ij = x_i - x_j is formed in FP64 and rounded once to FP32
 //     (dsub32), so the separation is accurate to 1/2 ulp of itself;
 //   * the rest of the pair arithmetic is FP32 with v_rsq_f32;
 //   * sums run in FP32 over a tile of kTile j's and are folded into FP64
 //     accumulators after every tile (block-compensated summation);
-//   * near field: a pair inside its near-field radius (r^2 < near2_j, see
-//     near_scale_kernel) contributes nothing in FP32; after each tile the
-//     engine re-evaluates exactly those pairs with the FP64 policy from the
-//     caller's FP64 arrays.  The few largest pair forces of each particle --
-//     the ones whose FP32 rounding would dominate its error when they
-//     nearly cancel -- are thereby exact to FP64.
+//   * near field: a pair inside its near-field radius (see ctx_kernel)
+//     contributes nothing in FP32; after each tile the engine re-evaluates
+//     exactly those pairs with the FP64 policy from the caller's FP64
+//     arrays.  The few largest pair forces of each particle -- the ones
+//     whose FP32 rounding would dominate its error when they nearly cancel
+//     -- are thereby exact to FP64.
 //
 // Pair formulas and semantics follow the reference cores:
 //   Phi         tupan/lib/src/phi_kernel_common.h:7-31
@@ -59,9 +60,9 @@ struct Pos32cJ { double x, y, z; float m, e2, near2, pad; };
 struct Pos32cI { double x, y, z; float e2; int i; };
 
 __device__ __forceinline__ Pos32cJ pack_pos32c(const Soa &s, int j,
-                                               double sc) {
+                                               const PackCtx &c) {
   return {s.f[RX][j], s.f[RY][j], s.f[RZ][j], (float)s.f[M][j],
-          (float)s.f[E2][j], near2_of(s.f[M][j], sc), 0.0f};
+          (float)s.f[E2][j], near_threshold(s.f[M][j], s.f[E2][j], c), 0.0f};
 }
 __device__ __forceinline__ Pos32cI load_pos32c(const Soa &s, int i) {
   return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], (float)s.f[E2][i], i};
@@ -71,15 +72,15 @@ struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
   static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using JRec = Pos32cJ;
   using I = Pos32cI;
-  __device__ static JRec pack(const Soa &s, int j, double sc) {
-    return pack_pos32c(s, j, sc);
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
+    return pack_pos32c(s, j, c);
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return load_pos32c(s, i);
   }
-  __device__ static void body32(const I &q, const JRec &r, float, float,
-                                float, float r2, bool far, A32 &a) {
-    float y = rsq32(r2 + (q.e2 + r.e2));
+  __device__ static void body32(const I &, const JRec &r, float, float,
+                                float, float s2, bool far, A32 &a) {
+    float y = rsq32(s2);
     y = far ? y : 0.0f;
     a.v[0] -= r.m * y;
   }
@@ -139,15 +140,15 @@ struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
   static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using JRec = Pos32cJ;
   using I = Pos32cI;
-  __device__ static JRec pack(const Soa &s, int j, double sc) {
-    return pack_pos32c(s, j, sc);
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
+    return pack_pos32c(s, j, c);
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return load_pos32c(s, i);
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
-                                float rz, float r2, bool far, A32 &a) {
-    float y = rsq32(r2 + (q.e2 + r.e2));
+                                float rz, float s2, bool far, A32 &a) {
+    float y = rsq32(s2);
     y = far ? y : 0.0f;
     float k = r.m * y * (y * y);
     a.v[0] -= k * rx;
@@ -199,27 +200,47 @@ struct AccJerkF64 : SumOut<6> {
 
 struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
   static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
-  // velocities stay FP64 like positions: rounding v_i to FP32 would add the
-  // same error to every pair of particle i, and that coherent term
-  // (dv_i * sum_j m_j/s^3) does not average out in the jerk
-  struct JRec { double x, y, z, vx, vy, vz; float m, e2, near2, pad; };
-  struct I { double x, y, z, vx, vy, vz; float e2; int i; };
-  __device__ static JRec pack(const Soa &s, int j, double sc) {
-    return {s.f[RX][j],       s.f[RY][j],        s.f[RZ][j],
-            s.f[VX][j],       s.f[VY][j],        s.f[VZ][j],
-            (float)s.f[M][j], (float)s.f[E2][j], near2_of(s.f[M][j], sc),
-            0.0f};
+  // Velocity differences from FP32 offsets to the j tile's FP64 velocity
+  // origin: the j offset is rounded once in pack, the i offset once per
+  // tile (tile_begin).  Rounding v_i itself to FP32 would add the same error
+  // to every pair of particle i, and that coherent term
+  // (dv_i * sum_j m_j/s^3) would not average out in the jerk; per tile it
+  // is shared by 32 pairs only.
+  static constexpr int kVelOrigins = 1;
+  struct JRec { double x, y, z; float dvx, dvy, dvz, m, e2, near2; };
+  struct I {
+    double x, y, z, vx, vy, vz;
+    float e2, dvx, dvy, dvz;
+    int i;
+  };
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
+    const double *o = c.origins + 4 * (j / kTile);
+    return {s.f[RX][j],
+            s.f[RY][j],
+            s.f[RZ][j],
+            dsub32(s.f[VX][j], o[0]),
+            dsub32(s.f[VY][j], o[1]),
+            dsub32(s.f[VZ][j], o[2]),
+            (float)s.f[M][j],
+            (float)s.f[E2][j],
+            near_threshold(s.f[M][j], s.f[E2][j], c)};
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
-    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i],        s.f[VX][i],
-            s.f[VY][i], s.f[VZ][i], (float)s.f[E2][i], i};
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i], s.f[VY][i],
+            s.f[VZ][i], (float)s.f[E2][i], 0.0f, 0.0f, 0.0f, i};
+  }
+  // the i-side velocity offsets to tile t's origin
+  __device__ static void tile_begin(I &q, const double *origins, int t) {
+    const double *o = origins + 4 * t;
+    q.dvx = dsub32(q.vx, o[0]);
+    q.dvy = dsub32(q.vy, o[1]);
+    q.dvz = dsub32(q.vz, o[2]);
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
-                                float rz, float r2, bool far, A32 &a) {
-    float vx = dsub32(q.vx, r.vx), vy = dsub32(q.vy, r.vy),
-          vz = dsub32(q.vz, r.vz);
+                                float rz, float s2, bool far, A32 &a) {
+    float vx = q.dvx - r.dvx, vy = q.dvy - r.dvy, vz = q.dvz - r.dvz;
     float rv = rx * vx + ry * vy + rz * vz;
-    float y = rsq32(r2 + (q.e2 + r.e2));
+    float y = rsq32(s2);
     y = far ? y : 0.0f;
     float y2 = y * y;
     float k = r.m * y * y2;
@@ -323,12 +344,12 @@ struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
     float e2;
     int i;
   };
-  __device__ static JRec pack(const Soa &s, int j, double sc) {
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
     return {s.f[RX][j],       s.f[RY][j],        s.f[RZ][j], s.f[VX][j],
             s.f[VY][j],       s.f[VZ][j],        s.f[AX][j], s.f[AY][j],
             s.f[AZ][j],       s.f[JX][j],        s.f[JY][j], s.f[JZ][j],
-            (float)s.f[M][j], (float)s.f[E2][j], near2_of(s.f[M][j], sc),
-            0.0f};
+            (float)s.f[M][j], (float)s.f[E2][j],
+            near_threshold(s.f[M][j], s.f[E2][j], c), 0.0f};
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i],
@@ -337,8 +358,8 @@ struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
             (float)s.f[E2][i], i};
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
-                                float rz, float r2, bool far, A32 &a) {
-    float y = rsq32(r2 + (q.e2 + r.e2));
+                                float rz, float s2, bool far, A32 &a) {
+    float y = rsq32(s2);
     y = far ? y : 0.0f;
     snap_crackle_pair<float>(
         rx, ry, rz, dsub32(q.vx, r.vx), dsub32(q.vy, r.vy),

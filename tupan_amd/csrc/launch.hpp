@@ -43,10 +43,19 @@ template <class Pol, int P>
 static void launch_p(dim3 grid, hipStream_t st, const Soa &is, const Soa &js,
                      int ni, const typename Pol::JRec *jr, int nj, int jchunk,
                      double *part, const Out &out,
-                     const typename Pol::Params &prm) {
+                     const typename Pol::Params &prm, const double *aux) {
   hipLaunchKernelGGL((pair_engine<Pol, P>), grid, dim3(kBlock), 0, st, is,
-                     js, ni, jr, nj, jchunk, part, out, prm);
+                     js, ni, jr, nj, jchunk, part, out, prm, aux);
 }
+
+template <class Pol, class = void>
+struct VelOrigins {
+  static constexpr int value = 0;
+};
+template <class Pol>
+struct VelOrigins<Pol, decltype((void)Pol::kVelOrigins)> {
+  static constexpr int value = Pol::kVelOrigins;
+};
 
 template <class Pol>
 int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
@@ -83,16 +92,20 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
 
   typename Pol::JRec *jr = nullptr;
   double *part = nullptr;
-  double *scale = nullptr;
+  double *ctx = nullptr;  // FP32C: factor, e2max, per-tile origins
   int rc;
+  const size_t ntiles = ((size_t)n_j + kTile - 1) / kTile;
+  const size_t rec_bytes =
+      ((sizeof(typename Pol::JRec) * (size_t)n_j + 255) / 256) * 256;
+  const size_t ctx_bytes =
+      Pol::kTwoLevel ? sizeof(double) * (2 + 4 * ntiles + 2 * kCtxBlocks)
+                     : 0;
   if (n_j > 0) {
-    rc = device_alloc((void **)&jr,
-                      sizeof(typename Pol::JRec) * (size_t)n_j + 256, st);
+    rc = device_alloc((void **)&jr, rec_bytes + ctx_bytes, st);
     if (rc) return rc;
-    // the near-field factor lives in the 256 bytes after the records
-    scale = reinterpret_cast<double *>(
-        reinterpret_cast<char *>(jr) +
-        ((sizeof(typename Pol::JRec) * (size_t)n_j + 127) / 128) * 128);
+    if (Pol::kTwoLevel)
+      ctx = reinterpret_cast<double *>(reinterpret_cast<char *>(jr) +
+                                       rec_bytes);
   }
   if (S > 1) {
     rc = device_alloc((void **)&part,
@@ -104,30 +117,37 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     }
   }
   if (n_j > 0) {
-    const bool near = Pol::kTwoLevel && near_field_factor() > 0.0;
-    if (near)
-      hipLaunchKernelGGL(near_scale_kernel, dim3(1), dim3(1024), 0, st,
-                         js.f[M], n_j, near_field_factor(), scale);
+    if (Pol::kTwoLevel) {
+      double *scratch = ctx + 2 + 4 * ntiles;
+      hipLaunchKernelGGL(ctx_partial_kernel, dim3(kCtxBlocks), dim3(256), 0,
+                         st, js.f[M], n_j, is.f[E2], n_i, scratch);
+      hipLaunchKernelGGL(ctx_final_kernel, dim3(1), dim3(64), 0, st, scratch,
+                         n_j, near_field_factor(), ctx);
+      if (VelOrigins<Pol>::value)
+        hipLaunchKernelGGL(tile_origin_kernel,
+                           dim3((unsigned)((ntiles + 255) / 256)), dim3(256),
+                           0, st, js.f[VX], js.f[VY], js.f[VZ], n_j, ctx + 2);
+    }
     hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
-                       dim3(256), 0, st, js, n_j, jr,
-                       near ? (const double *)scale : nullptr);
+                       dim3(256), 0, st, js, n_j, jr, (const double *)ctx);
   }
   const int waves = (n_i + 64 * P - 1) / (64 * P);
   dim3 grid((waves + kWaves - 1) / kWaves, S);
+  const double *aux = ctx ? ctx + 2 : nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   const bool timed = timing_begin(&ev0, &ev1);
   if (timed) (void)hipEventRecord(ev0, st);
   switch (P) {
     case 4:
       launch_p<Pol, (Pol::kMaxIpl >= 4 ? 4 : 1)>(grid, st, is, js, n_i, jr, n_j,
-                                                  jchunk, part, out, prm);
+                                                  jchunk, part, out, prm, aux);
       break;
     case 2:
       launch_p<Pol, (Pol::kMaxIpl >= 2 ? 2 : 1)>(grid, st, is, js, n_i, jr, n_j,
-                                                  jchunk, part, out, prm);
+                                                  jchunk, part, out, prm, aux);
       break;
     default:
-      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, part, out, prm);
+      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, part, out, prm, aux);
   }
   if (timed) (void)hipEventRecord(ev1, st);
   if (S > 1)
