@@ -13,7 +13,8 @@ import numpy as np
 from . import spec
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libtupan_hip.so")
+LIB_PATH = os.environ.get("TUPAN_HIP_LIB",
+                          os.path.join(HERE, "lib", "libtupan_hip.so"))
 
 MODE_FP64 = 0
 MODE_FP32C = 1
