@@ -101,6 +101,33 @@ void sakura_kernel(const UINT ni, const REAL *_im, const REAL *_irx,
                    const REAL dt, const INT flag, REAL *_idrx, REAL *_idry,
                    REAL *_idrz, REAL *_idvx, REAL *_idvy, REAL *_idvz);
 
+/* libtupan.h:152-179 */
+void nreg_Xkernel(const UINT ni, const REAL *_im, const REAL *_irx,
+                  const REAL *_iry, const REAL *_irz, const REAL *_ie2,
+                  const REAL *_ivx, const REAL *_ivy, const REAL *_ivz,
+                  const UINT nj, const REAL *_jm, const REAL *_jrx,
+                  const REAL *_jry, const REAL *_jrz, const REAL *_je2,
+                  const REAL *_jvx, const REAL *_jvy, const REAL *_jvz,
+                  const REAL dt, REAL *_idrx, REAL *_idry, REAL *_idrz,
+                  REAL *_iax, REAL *_iay, REAL *_iaz, REAL *_iu);
+
+/* libtupan.h:181-201 */
+void nreg_Vkernel(const UINT ni, const REAL *_im, const REAL *_ivx,
+                  const REAL *_ivy, const REAL *_ivz, const REAL *_iax,
+                  const REAL *_iay, const REAL *_iaz, const UINT nj,
+                  const REAL *_jm, const REAL *_jvx, const REAL *_jvy,
+                  const REAL *_jvz, const REAL *_jax, const REAL *_jay,
+                  const REAL *_jaz, const REAL dt, REAL *_idvx,
+                  REAL *_idvy, REAL *_idvz, REAL *_ik);
+
+/* libtupan.h:231-246 -- particles 0 and 1; outputs may alias inputs */
+void kepler_solver_kernel(const REAL *_im, const REAL *_irx,
+                          const REAL *_iry, const REAL *_irz,
+                          const REAL *_ie2, const REAL *_ivx,
+                          const REAL *_ivy, const REAL *_ivz, const REAL dt,
+                          REAL *_ir1x, REAL *_ir1y, REAL *_ir1z,
+                          REAL *_iv1x, REAL *_iv1y, REAL *_iv1z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,7 +61,9 @@ extern "C" {
 #define TUPAN_K_SAKURA 6
 #define TUPAN_K_ACC_ADJOINT 7
 #define TUPAN_K_PHI_ADJOINT 8
-#define TUPAN_K_COUNT 9
+#define TUPAN_K_NREG_X 9
+#define TUPAN_K_NREG_V 10
+#define TUPAN_K_COUNT 11
 
 /* ---- runtime ------------------------------------------------------- */
 const char *tupan_version(void);
@@ -149,6 +151,33 @@ int tupan_sakura_dev(size_t ni, const double *const *ip, size_t nj,
 int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
                       const double *const *jp, double dt, int flag,
                       double *const *op);
+
+/* ---- NREG X: replaces nreg_Xkernel, libtupan.h:152-179 ---------------
+ * ip, jp: F8.  op: mrx, mry, mrz, ax, ay, az, u -- sums over j at the drifted
+ * separation r_ij + v_ij dt of m_j r, the softened acceleration, and
+ * u_i = m_i sum_j m_j / s (FP64). */
+int tupan_nreg_x_dev(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, double *const *op,
+                     void *stream);
+int tupan_nreg_x_host(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double dt, double *const *op);
+
+/* ---- NREG V: replaces nreg_Vkernel, libtupan.h:181-201 ---------------
+ * ip, jp: m, vx, vy, vz, ax, ay, az.  op: mvx, mvy, mvz, mk -- sums over j
+ * of m_j w and k_i = m_i sum_j m_j |w|^2 with w = v_ij + a_ij dt (FP64). */
+int tupan_nreg_v_dev(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, double *const *op,
+                     void *stream);
+int tupan_nreg_v_host(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double dt, double *const *op);
+
+/* ---- Kepler: replaces kepler_solver_kernel, libtupan.h:231-246 -------
+ * ip: F8 arrays of which particles 0 and 1 are used; op: rx, ry, rz, vx,
+ * vy, vz of the two particles after dt (may alias ip's fields, as the
+ * reference updates in place).  FP64; d_status as for Sakura. */
+int tupan_kepler_dev(const double *const *ip, double dt, double *const *op,
+                     int *d_status, void *stream);
+int tupan_kepler_host(const double *const *ip, double dt, double *const *op);
 
 /* ---- pair-selected variants for hybrid schemes -------------------------
  * No counterpart in the reference (which applies Sakura to every pair).

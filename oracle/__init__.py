@@ -13,10 +13,10 @@ Two kinds of library sit behind one call interface:
   C kernels (tupan/lib/src/*.c) compiled from source by ``make ref``; present
   only where the reference checkout was available at build time.
 
-Argument lists follow tupan/lib/src/libtupan.h:2-229; the i/j field order of
+Argument lists follow tupan/lib/src/libtupan.h:2-246; the i/j field order of
 each kernel follows tupan/lib/extensions.py (Phi :126-131, Acc :192-197,
 AccJerk :273-278, SnapCrackle :329-336, Tstep :375-381, PNAcc :424-433,
-Sakura :494-500).
+Sakura :494-500, NREG_X :551-558, NREG_V :602-608, Kepler :642-646).
 """
 import ctypes
 import os
@@ -39,7 +39,15 @@ SPECS = {
     "pnacc_kernel": (F8, ("uint",) + ("real",) * 7, ("pnax", "pnay", "pnaz")),
     "sakura_kernel": (F8, ("real", "int"),
                       ("drx", "dry", "drz", "dvx", "dvy", "dvz")),
+    "nreg_Xkernel": (F8, ("real",),
+                     ("mrx", "mry", "mrz", "ax", "ay", "az", "u")),
+    "nreg_Vkernel": (("mass", "vx", "vy", "vz", "ax", "ay", "az"), ("real",),
+                     ("mvx", "mvy", "mvz", "mk")),
 }
+
+# kepler_solver_kernel (libtupan.h:231-246) has no ni/nj: it advances
+# particles 0 and 1 of the given arrays.
+KEPLER = (F8, ("rx", "ry", "rz", "vx", "vy", "vz"))
 
 
 def lib_path(prec="float64", kind="port"):
@@ -106,6 +114,19 @@ class Oracle(object):
         fn = getattr(self.lib, self.prefix + name)
         fn.restype = None
         fn(*args)
+        return tuple(res)
+
+    def kepler(self, ps, dt):
+        """kepler_solver_kernel on particles 0 and 1 of ``ps``: returns the
+        new (rx, ry, rz, vx, vy, vz) arrays of length 2."""
+        fields, outs = KEPLER
+        arrs = [self._arr(_get(ps, f))[:2].copy() for f in fields]
+        res = [np.zeros(2, dtype=self.real) for _ in outs]
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        fn = getattr(self.lib, self.prefix + "kepler_solver_kernel")
+        fn.restype = None
+        fn(*([ptr(a) for a in arrs] + [self.c_real(dt)]
+             + [ptr(a) for a in res]))
         return tuple(res)
 
     def twobody_evolve(self, dt, flag, m, e2, state):

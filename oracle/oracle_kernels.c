@@ -1017,3 +1017,123 @@ EXPORT void oracle_twobody_evolve(real dt, sint flag, real m, real e2,
 {
     pair_evolve(dt, flag, m, e2, state6);
 }
+
+/* ------------------------------------------------------------- NREG X */
+/* nreg_kernels_common.h:7-58, driver nreg_kernels.c:5-61: pair sums at the
+ * drifted separation r + v dt; u is finished as m_i * sum_j m_j / s. */
+EXPORT void oracle_nreg_Xkernel(
+    const uint_t ni, RP im, RP irx, RP iry, RP irz, RP ie2,
+    RP ivx, RP ivy, RP ivz,
+    const uint_t nj, RP jm, RP jrx, RP jry, RP jrz, RP je2,
+    RP jvx, RP jvy, RP jvz,
+    const real dt,
+    WP idrx, WP idry, WP idrz, WP iax, WP iay, WP iaz, WP iu)
+{
+    for (uint_t i = 0; i < ni; ++i) {
+        real drx = 0, dry = 0, drz = 0, ax = 0, ay = 0, az = 0, u = 0;
+        for (uint_t j = 0; j < nj; ++j) {
+            real rx = irx[i] - jrx[j];
+            real ry = iry[i] - jry[j];
+            real rz = irz[i] - jrz[j];
+            real e2 = ie2[i] + je2[j];
+            real vx = ivx[i] - jvx[j];
+            real vy = ivy[i] - jvy[j];
+            real vz = ivz[i] - jvz[j];
+            rx += vx * dt;
+            ry += vy * dt;
+            rz += vz * dt;
+            real r2 = rx * rx + ry * ry + rz * rz;
+            real inv_r2 = soft_inv_r2(r2, e2);
+            real inv_r1 = R_SQRT(inv_r2);
+            real inv_r3 = inv_r2 * inv_r1;
+            inv_r3 *= jm[j];
+            drx += jm[j] * rx;
+            dry += jm[j] * ry;
+            drz += jm[j] * rz;
+            ax -= inv_r3 * rx;
+            ay -= inv_r3 * ry;
+            az -= inv_r3 * rz;
+            u += jm[j] * inv_r1;
+        }
+        idrx[i] = drx; idry[i] = dry; idrz[i] = drz;
+        iax[i] = ax; iay[i] = ay; iaz[i] = az;
+        iu[i] = im[i] * u;
+    }
+}
+
+/* ------------------------------------------------------------- NREG V */
+/* nreg_kernels_common.h:61-99, driver nreg_kernels.c:64-115: pair sums at
+ * the kicked relative velocity v + a dt; k is finished as m_i * sum. */
+EXPORT void oracle_nreg_Vkernel(
+    const uint_t ni, RP im, RP ivx, RP ivy, RP ivz, RP iax, RP iay, RP iaz,
+    const uint_t nj, RP jm, RP jvx, RP jvy, RP jvz, RP jax, RP jay, RP jaz,
+    const real dt,
+    WP idvx, WP idvy, WP idvz, WP ik)
+{
+    for (uint_t i = 0; i < ni; ++i) {
+        real dvx = 0, dvy = 0, dvz = 0, k = 0;
+        for (uint_t j = 0; j < nj; ++j) {
+            real vx = ivx[i] - jvx[j];
+            real vy = ivy[i] - jvy[j];
+            real vz = ivz[i] - jvz[j];
+            real ax = iax[i] - jax[j];
+            real ay = iay[i] - jay[j];
+            real az = iaz[i] - jaz[j];
+            vx += ax * dt;
+            vy += ay * dt;
+            vz += az * dt;
+            real v2 = vx * vx + vy * vy + vz * vz;
+            dvx += jm[j] * vx;
+            dvy += jm[j] * vy;
+            dvz += jm[j] * vz;
+            k += jm[j] * v2;
+        }
+        idvx[i] = dvx; idvy[i] = dvy; idvz[i] = dvz;
+        ik[i] = im[i] * k;
+    }
+}
+
+/* ------------------------------------------------------------- Kepler */
+/* kepler_solver_kernel_common.h:8-79, driver kepler_solver_kernel.c:5-49:
+ * particles 0 and 1 advanced by dt along their (softened) Kepler orbit
+ * about the centre of mass, which drifts freely. */
+EXPORT void oracle_kepler_solver_kernel(
+    RP m, RP rx, RP ry, RP rz, RP e2_, RP vx, RP vy, RP vz,
+    const real dt,
+    WP r1x, WP r1y, WP r1z, WP v1x, WP v1y, WP v1z)
+{
+    real r0x = rx[0] - rx[1];
+    real r0y = ry[0] - ry[1];
+    real r0z = rz[0] - rz[1];
+    real e2 = e2_[0] + e2_[1];
+    real v0x = vx[0] - vx[1];
+    real v0y = vy[0] - vy[1];
+    real v0z = vz[0] - vz[1];
+    real mt = m[0] + m[1];
+    real imu = m[0] / mt;
+    real jmu = m[1] / mt;
+    real rcmx = imu * rx[0] + jmu * rx[1];
+    real rcmy = imu * ry[0] + jmu * ry[1];
+    real rcmz = imu * rz[0] + jmu * rz[1];
+    real vcmx = imu * vx[0] + jmu * vx[1];
+    real vcmy = imu * vy[0] + jmu * vy[1];
+    real vcmz = imu * vz[0] + jmu * vz[1];
+    rcmx += vcmx * dt;
+    rcmy += vcmy * dt;
+    rcmz += vcmz * dt;
+    real in[6] = {r0x, r0y, r0z, v0x, v0y, v0z};
+    real o[6];
+    kepler_drift(dt, mt, e2, in, o);
+    r1x[0] = rcmx + jmu * o[0];
+    r1y[0] = rcmy + jmu * o[1];
+    r1z[0] = rcmz + jmu * o[2];
+    v1x[0] = vcmx + jmu * o[3];
+    v1y[0] = vcmy + jmu * o[4];
+    v1z[0] = vcmz + jmu * o[5];
+    r1x[1] = rcmx - imu * o[0];
+    r1y[1] = rcmy - imu * o[1];
+    r1z[1] = rcmz - imu * o[2];
+    v1x[1] = vcmx - imu * o[3];
+    v1y[1] = vcmy - imu * o[4];
+    v1z[1] = vcmz - imu * o[5];
+}

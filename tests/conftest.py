@@ -116,4 +116,7 @@ OUT_GROUPS = {
     "tstep": [(0,), (1,)],
     "pnacc": [(0, 1, 2)],
     "sakura": [(0, 1, 2), (3, 4, 5)],
+    "nreg_x": [(0, 1, 2), (3, 4, 5), (6,)],
+    "nreg_v": [(0, 1, 2), (3,)],
+    "kepler": [(0, 1, 2), (3, 4, 5)],
 }

@@ -11,7 +11,8 @@ shapes (tupan/tests/test_extensions.py:39-127: random sets, prefix slices
 ps[:k] on either side, eta = 1/64, PN order 7 at clight = 128) and add the
 edge cases that test leaves open (coincident particles, softened sets, PN
 orders 0..7, every Sakura flag with close, wide, bound and unbound pairs,
-the Plummer N=256 seed-0 model).  For every FP64 sum kernel the file also
+the Plummer N=256 seed-0 model, the NREG pair sums at several dt and the
+two-body Kepler solver on bound, unbound and softened pairs).  For every FP64 sum kernel the file also
 holds, per particle and output vector, S_i = sum_j ||f_ij||, the magnitude
 scale of its pair terms (the condition of the sum), from the same library.
 """
@@ -34,7 +35,8 @@ F14 = SPECS["snap_crackle_kernel"][0]
 SHORT = {"phi_kernel": "phi", "acc_kernel": "acc",
          "acc_jerk_kernel": "acc_jerk", "snap_crackle_kernel": "snap_crackle",
          "tstep_kernel": "tstep", "pnacc_kernel": "pnacc",
-         "sakura_kernel": "sakura"}
+         "sakura_kernel": "sakura", "nreg_Xkernel": "nreg_x",
+         "nreg_Vkernel": "nreg_v", "kepler_solver_kernel": "kepler"}
 
 
 def random_set(seed, n, soft):
@@ -89,7 +91,9 @@ def plummer_set(ref):
 GROUPS = {"phi": [(0,)], "acc": [(0, 1, 2)],
           "acc_jerk": [(0, 1, 2), (3, 4, 5)],
           "snap_crackle": [(0, 1, 2), (3, 4, 5)],
-          "pnacc": [(0, 1, 2)], "sakura": [(0, 1, 2), (3, 4, 5)]}
+          "pnacc": [(0, 1, 2)], "sakura": [(0, 1, 2), (3, 4, 5)],
+          "nreg_x": [(0, 1, 2), (3, 4, 5), (6,)],
+          "nreg_v": [(0, 1, 2), (3,)]}
 
 
 def abs_sums(lib, name, ips, jps, args):
@@ -167,6 +171,27 @@ def main():
     add("sakura_kernel", "K1", "float32", (1.0 / 64, -1))
     add("sakura_kernel", "K1", "float32", (1.0 / 64, 2))
 
+    # NREG pair sums at drifted / kicked separations (libtupan.h:152-201)
+    for s_, dt in (("R", 1.0 / 64), ("S", 1.0 / 64), ("P", 1.0 / 64),
+                   ("S", 0.0), ("P", -0.125)):
+        add("nreg_Xkernel", s_, "float64", (dt,))
+        add("nreg_Vkernel", s_, "float64", (dt,))
+    for k in (1, 63, 65):
+        add("nreg_Xkernel", "R", "float64", (1.0 / 64,), jpre=k)
+        add("nreg_Vkernel", "R", "float64", (1.0 / 64,), jpre=k)
+    for s_ in ("R", "P"):
+        add("nreg_Xkernel", s_, "float32", (1.0 / 64,))
+        add("nreg_Vkernel", s_, "float32", (1.0 / 64,))
+    # Kepler two-body solver on particles 0 and 1 (libtupan.h:231-246):
+    # K0/K1 hold a tight bound binary there, R an unbound pair, P a wide
+    # Plummer pair; softened K1 only at small dt (see Sakura above)
+    for s_, dts in (("K0", (1.0 / 64, 0.25, -0.1)), ("K1", (1.0 / 64, 1.0 / 128)),
+                    ("R", (1.0 / 64, 0.25)), ("P", (1.0 / 64, 0.25))):
+        for dt in dts:
+            add("kepler_solver_kernel", s_, "float64", (dt,))
+    add("kepler_solver_kernel", "K0", "float32", (1.0 / 64,))
+    add("kepler_solver_kernel", "R", "float32", (0.25,))
+
     out = {}
     for sname, d in sets.items():
         for f, v in d.items():
@@ -179,7 +204,10 @@ def main():
         if cs["prec"] == "float32":
             ips = {f: v.astype(np.float32) for f, v in ips.items()}
             jps = {f: v.astype(np.float32) for f, v in jps.items()}
-        res = lib.call(cs["kernel"], ips, jps, *cs["args"])
+        if cs["kernel"] == "kepler_solver_kernel":
+            res = lib.kepler(ips, *cs["args"])
+        else:
+            res = lib.call(cs["kernel"], ips, jps, *cs["args"])
         if cs["prec"] == "float64" and SHORT[cs["kernel"]] in GROUPS:
             for g, v in enumerate(abs_sums(lib, cs["kernel"], ips, jps,
                                            cs["args"])):

@@ -34,7 +34,11 @@ from tupan_amd import _lib, kernels, spec  # noqa: E402
 # dt-proportional) size.  FP32C: 1e-6 relative / ~3 FP32 ulps of S.
 FP64_TOL = {"phi": (1e-12, 1e-14), "acc": (1e-12, 1e-14),
             "acc_jerk": (1e-12, 1e-14), "snap_crackle": (1e-12, 1e-14),
-            "pnacc": (1e-12, 1e-14), "sakura": (1e-12, 1e-10)}
+            "pnacc": (1e-12, 1e-14), "sakura": (1e-12, 1e-10),
+            "nreg_x": (1e-12, 1e-14), "nreg_v": (1e-12, 1e-14)}
+# Kepler: the Laguerre solve stops at a 2^-42 step: new states agree to
+# ~1e-13 of the orbit's size
+KEPLER_TOL = 1e-11
 TSTEP_TOL = 1e-12
 FP32C_TOL = (1e-6, 2e-7)
 # SnapCrackle's FP32 pair recurrences (alpha, beta, gamma) round ~10 times
@@ -87,8 +91,15 @@ def test_golden_fp64(golden):
             continue
         k = case["kernel"]
         ip, jp = golden.inputs(case)
-        got = kernels.compute(k, ip, jp, *golden.args(case), mode="fp64")
         ref = golden.outputs(case)
+        if k == "kepler":
+            got = kernels.kepler(ip, *golden.args(case))
+            e = max(np.abs(g - r).max() / (np.abs(r).max() + 1e-300)
+                    for g, r in zip(got, ref)) / KEPLER_TOL
+            worst[k] = max(worst.get(k, 0.0), e)
+            assert e <= 1.0, (case, e)
+            continue
+        got = kernels.compute(k, ip, jp, *golden.args(case), mode="fp64")
         if k == "tstep":
             e = _err(k, got, ref) / TSTEP_TOL
         elif k == "sakura":
@@ -126,7 +137,12 @@ def _load_dropin(bits):
                                     "libtupan_f%s.so" % bits))
 
 
+DROPIN_SYMBOL = {"nreg_x": "nreg_Xkernel", "nreg_v": "nreg_Vkernel"}
+
+
 def _call_dropin(lib, name, ip, jp, args, real):
+    if name == "kepler":
+        return _call_dropin_kepler(lib, ip, args[0], real)
     k = spec.KERNELS[name]
     ia = [np.ascontiguousarray(ip[f], real) for f in k.fields]
     ja = [np.ascontiguousarray(jp[f], real) for f in k.fields]
@@ -143,12 +159,28 @@ def _call_dropin(lib, name, ip, jp, args, real):
         cargs += [c_uint(args[0])] + [c_real(x) for x in args[1]]
     elif name == "sakura":
         cargs += [c_real(args[0]), c_int(args[1])]
+    elif name in ("nreg_x", "nreg_v"):
+        cargs.append(c_real(args[0]))
     cargs += [p(o) for o in outs]
-    fn = getattr(lib, name + "_kernel")
+    fn = getattr(lib, DROPIN_SYMBOL.get(name, name + "_kernel"))
     fn.restype = None
     fn(*cargs)
     assert lib.tupan_abi_last_status() == 0
     return outs
+
+
+def _call_dropin_kepler(lib, ps, dt, real):
+    """In place, as the reference's Kepler extension calls it."""
+    arrs = {f: np.ascontiguousarray(ps[f], real).copy()
+            for f in spec.KEPLER_FIELDS}
+    c_real = ctypes.c_double if real == np.float64 else ctypes.c_float
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    fn = lib.kepler_solver_kernel
+    fn.restype = None
+    fn(*([p(arrs[f]) for f in spec.KEPLER_FIELDS] + [c_real(dt)]
+         + [p(arrs[f]) for f in spec.KEPLER_OUTS]))
+    assert lib.tupan_abi_last_status() == 0
+    return [arrs[f][:2] for f in spec.KEPLER_OUTS]
 
 
 def test_dropin_f64_matches_c_abi(golden):
@@ -159,8 +191,11 @@ def test_dropin_f64_matches_c_abi(golden):
         ip, jp = golden.inputs(case)
         a = _call_dropin(lib, case["kernel"], ip, jp, golden.args(case),
                          np.float64)
-        b = kernels.compute(case["kernel"], ip, jp, *golden.args(case),
-                            mode="fp64")
+        if case["kernel"] == "kepler":
+            b = kernels.kepler(ip, *golden.args(case))
+        else:
+            b = kernels.compute(case["kernel"], ip, jp, *golden.args(case),
+                                mode="fp64")
         for x, y in zip(a, b):
             assert np.array_equal(x, y, equal_nan=True), case
 
@@ -177,8 +212,11 @@ def test_dropin_f32_at_least_as_accurate_as_reference_f32(golden):
         # FP64 truth on the same float32-rounded inputs
         ip64 = {f: v.astype(np.float64) for f, v in ip.items()}
         jp64 = {f: v.astype(np.float64) for f, v in jp.items()}
-        truth = kernels.compute(case["kernel"], ip64, jp64, *golden.args(case),
-                                mode="fp64")
+        if case["kernel"] == "kepler":
+            truth = kernels.kepler(ip64, *golden.args(case))
+        else:
+            truth = kernels.compute(case["kernel"], ip64, jp64,
+                                    *golden.args(case), mode="fp64")
         for o, r, t in zip(ours, ref32, truth):
             scale = np.max(np.abs(t)) + 1e-300
             e_ours = np.abs(o.astype(np.float64) - t) / scale

@@ -91,7 +91,11 @@ def test_no_oracle_import_in_product():
 
 
 def test_spec_matches_checker_specs():
+    assert len(spec.SYMBOLS) == 10     # every symbol of libtupan.h
     for sym, key in spec.SYMBOLS.items():
+        if key == "kepler":
+            assert (spec.KEPLER_FIELDS, spec.KEPLER_OUTS) == oracle.KEPLER
+            continue
         fields, _, outs = oracle.SPECS[sym]
         assert spec.KERNELS[key].fields == fields
         assert spec.KERNELS[key].outs == outs
@@ -148,6 +152,13 @@ class OracleKernel(extensions.HIPKernel):
     """Test backend: the reference protocol served by the CPU checker."""
 
     def run(self):
+        if self.key == "kepler":
+            a = self.args
+            ps = dict(zip(spec.KEPLER_FIELDS, a[:8]))
+            res = Oracle(self.prec, "port").kepler(ps, a[8])
+            for o, r in zip(a[9:15], res):
+                o[:2] = r
+            return
         k = spec.KERNELS[self.key]
         nf = len(k.fields)
         a = self.args
@@ -195,3 +206,31 @@ def test_extension_pnacc_uses_module_clight(golden):
 def test_unknown_backend_rejected():
     with pytest.raises(ValueError):
         extensions.Acc("CL", "float64")
+
+
+def test_extension_nreg_and_kepler_surface(golden):
+    """NREG_X / NREG_V register the reference's output names and take dt;
+    Kepler updates particles 0 and 1 in place (extensions.py:510-651)."""
+    d = golden.set("P")
+    ps = ParticleSystem(len(d["mass"]))
+    for f in spec.F8:
+        getattr(ps, f)[...] = d[f]
+    case = golden.find(kernel="nreg_x", set="P", prec="float64")[0]
+    res = extensions.NREG_X("ORACLE", "float64").calc(ps, ps, case["args"][0])
+    assert res[0] is ps.mrx and res[6] is ps.u
+    for got, ref in zip(res, golden.outputs(case)):
+        assert np.array_equal(got, ref)
+    for f in ("ax", "ay", "az"):
+        getattr(ps, f)[...] = d[f]
+    case = golden.find(kernel="nreg_v", set="P", prec="float64")[0]
+    res = extensions.NREG_V("ORACLE", "float64").calc(ps, ps, case["args"][0])
+    assert res[3] is ps.mk
+    for got, ref in zip(res, golden.outputs(case)):
+        assert np.array_equal(got, ref)
+    case = golden.find(kernel="kepler", set="P", prec="float64")[0]
+    rx = ps.rx
+    extensions.Kepler("ORACLE", "float64").calc(ps, ps, case["args"][0])
+    assert ps.rx is rx
+    for f, ref in zip(spec.KEPLER_OUTS, golden.outputs(case)):
+        assert np.array_equal(getattr(ps, f)[:2], ref)
+        assert np.array_equal(getattr(ps, f)[2:], d[f][2:])

@@ -32,16 +32,24 @@ def o32():
 
 SYM = {"phi": "phi_kernel", "acc": "acc_kernel", "acc_jerk": "acc_jerk_kernel",
        "snap_crackle": "snap_crackle_kernel", "tstep": "tstep_kernel",
-       "pnacc": "pnacc_kernel", "sakura": "sakura_kernel"}
+       "pnacc": "pnacc_kernel", "sakura": "sakura_kernel",
+       "nreg_x": "nreg_Xkernel", "nreg_v": "nreg_Vkernel"}
+
+
+def call_case(lib, case, ip, jp, args):
+    if case["kernel"] == "kepler":
+        return lib.kepler(ip, *args)
+    return lib.call(SYM[case["kernel"]], ip, jp, *args)
 
 
 def test_port_matches_golden_bitwise(golden, o64, o32):
-    assert len(golden.cases) > 100
+    assert len(golden.cases) > 150
+    assert {c["kernel"] for c in golden.cases} >= set(SYM) | {"kepler"}
     for case in golden.cases:
         lib = o64 if case["prec"] == "float64" else o32
         dt = np.float64 if case["prec"] == "float64" else np.float32
         ip, jp = golden.inputs(case, dt)
-        got = lib.call(SYM[case["kernel"]], ip, jp, *golden.oracle_args(case))
+        got = call_case(lib, case, ip, jp, golden.oracle_args(case))
         ref = golden.outputs(case)
         assert len(got) == len(ref)
         for g, r in zip(got, ref):
@@ -65,11 +73,16 @@ def test_port_matches_reference_on_fresh_inputs():
                            ("snap_crackle_kernel", ()),
                            ("tstep_kernel", (0.03,)),
                            ("pnacc_kernel", (7,) + tuple(50.0**-k for k in range(1, 8))),
-                           ("sakura_kernel", (1.0 / 64, 2))):
+                           ("sakura_kernel", (1.0 / 64, 2)),
+                           ("nreg_Xkernel", (0.03,)),
+                           ("nreg_Vkernel", (-0.02,))):
             a = P.call(name, dd, dd, *args)
             b = R.call(name, dd, dd, *args)
             for x, y in zip(a, b):
                 assert np.array_equal(x, y, equal_nan=True), (prec, name)
+        for dt in (0.01, 0.7):
+            for x, y in zip(P.kepler(dd, dt), R.kepler(dd, dt)):
+                assert np.array_equal(x, y, equal_nan=True), (prec, "kepler")
 
 
 def two_body(m1=0.7, m2=0.3, e21=1e-3, e22=2e-3, seed=0):
@@ -294,3 +307,69 @@ def test_sakura_softened_energy_conserved(o64):
     out = o64.twobody_evolve(dt, 1, m, e2, st)
     out[:3] += out[3:] * dt
     assert abs(E(out) - E(st)) < 1e-10 * abs(E(st))
+
+
+def test_nreg_x_is_acc_phi_at_drifted_positions(o64):
+    """nreg_Xkernel(dt) = sums at r_ij + v_ij dt: the acceleration and
+    potential of the set drifted by v dt, and sum_j m_j (r_ij + v_ij dt)."""
+    rng = np.random.RandomState(7)
+    n = 40
+    d = {f: rng.normal(0, 1, n) for f in oracle.F8}
+    d["mass"] = rng.uniform(0.1, 1, n)
+    d["eps2"] = rng.uniform(1e-4, 1e-3, n)
+    dt = 0.37
+    mrx, mry, mrz, ax, ay, az, u = o64.call("nreg_Xkernel", d, d, dt)
+    dr = dict(d)
+    for c in "xyz":
+        dr["r" + c] = d["r" + c] + d["v" + c] * dt
+    acc = o64.call("acc_kernel", dr, dr)
+    phi = o64.call("phi_kernel", dr, dr)[0]
+    for got, want in zip((ax, ay, az), acc):
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    assert np.allclose(u, -d["mass"] * phi, rtol=1e-12)
+    M = d["mass"].sum()
+    for got, c in zip((mrx, mry, mrz), "xyz"):
+        x = dr["r" + c]
+        assert np.allclose(got, M * x - (d["mass"] * x).sum(), rtol=1e-11,
+                           atol=1e-11)
+
+
+def test_nreg_v_closed_form(o64):
+    """nreg_Vkernel(dt): sum_j m_j w_ij and m_i sum_j m_j |w_ij|^2 with
+    w_ij = v_ij + a_ij dt."""
+    rng = np.random.RandomState(8)
+    n = 33
+    d = {f: rng.normal(0, 1, n) for f in ("mass", "vx", "vy", "vz", "ax", "ay", "az")}
+    d["mass"] = rng.uniform(0.1, 1, n)
+    dt = -0.21
+    mvx, mvy, mvz, mk = o64.call("nreg_Vkernel", d, d, dt)
+    w = np.stack([d["v" + c] + d["a" + c] * dt for c in "xyz"])
+    W = w[:, :, None] - w[:, None, :]
+    m = d["mass"]
+    for k, got in enumerate((mvx, mvy, mvz)):
+        assert np.allclose(got, (m[None, :] * W[k]).sum(1), rtol=1e-12, atol=1e-13)
+    assert np.allclose(mk, m * (m[None, :] * (W**2).sum(0)).sum(1), rtol=1e-12)
+
+
+def test_kepler_solver_is_exact_two_body(o64):
+    """kepler_solver_kernel: the relative orbit follows exact elliptic
+    Kepler motion and the centre of mass drifts freely."""
+    m = np.array([0.7, 0.4])
+    r0 = np.array([0.6, -0.2, 0.1])
+    v0 = np.array([0.3, 1.2, -0.2])
+    com_r = np.array([1.0, 2.0, -0.5])
+    com_v = np.array([0.01, -0.03, 0.02])
+    mt = m.sum()
+    d = {"mass": m, "eps2": np.zeros(2)}
+    for k, c in enumerate("xyz"):
+        d["r" + c] = np.array([com_r[k] + m[1] / mt * r0[k], com_r[k] - m[0] / mt * r0[k]])
+        d["v" + c] = np.array([com_v[k] + m[1] / mt * v0[k], com_v[k] - m[0] / mt * v0[k]])
+    dt = 1.3
+    rx, ry, rz, vx, vy, vz = o64.kepler(d, dt)
+    R = np.array([rx, ry, rz])
+    V = np.array([vx, vy, vz])
+    r_rel, v_rel = kepler_exact(r0, v0, mt, dt)
+    assert np.allclose(R[:, 0] - R[:, 1], r_rel, atol=1e-11)
+    assert np.allclose(V[:, 0] - V[:, 1], v_rel, atol=1e-11)
+    assert np.allclose((R * m).sum(1) / mt, com_r + com_v * dt, atol=1e-13)
+    assert np.allclose((V * m).sum(1) / mt, com_v, atol=1e-13)

@@ -105,6 +105,13 @@ def lib():
         L.tupan_kick_pn_half_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
         L.tupan_pn_flux_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
         L.tupan_apply_delta_dev.argtypes = [_SZ, _PP, _PP, _P]
+        for name in ("nreg_x", "nreg_v"):
+            getattr(L, "tupan_%s_dev" % name).argtypes = [
+                _SZ, _PP, _SZ, _PP, _D, _PP, _P]
+            getattr(L, "tupan_%s_host" % name).argtypes = [
+                _SZ, _PP, _SZ, _PP, _D, _PP]
+        L.tupan_kepler_dev.argtypes = [_PP, _D, _PP, _P, _P]
+        L.tupan_kepler_host.argtypes = [_PP, _D, _PP]
         L.tupan_acc_far_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _PP, _P]
         L.tupan_sakura_near_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _I, _D,
                                             _PP, _P, _P]
@@ -186,7 +193,7 @@ def _cargs(name, mode, scalars):
         post.append(inv)  # keep alive; stripped before the call
     elif name == "sakura":
         post = [ctypes.c_double(float(scalars[0])), ctypes.c_int(int(scalars[1]))]
-    elif name == "acc_far":
+    elif name in ("acc_far", "nreg_x", "nreg_v"):
         post = [ctypes.c_double(float(scalars[0]))]
     elif name == "sakura_near":
         post = [ctypes.c_double(float(scalars[0])), ctypes.c_int(int(scalars[1])),
@@ -224,6 +231,13 @@ def call_dev(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=(), stream=0,
     rc = fn(*args)
     del keep
     check(rc)
+
+
+def call_kepler_host(iptrs, dt, optrs):
+    """kepler_solver_kernel on particles 0 and 1 (host pointers, in place
+    allowed)."""
+    check(lib().tupan_kepler_host(ptr_array(iptrs), ctypes.c_double(float(dt)),
+                                  ptr_array(optrs)))
 
 
 def call_update(name, *args):

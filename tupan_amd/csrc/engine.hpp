@@ -36,7 +36,7 @@ struct Soa {
   const double *f[14];
 };
 struct Out {
-  double *f[6];
+  double *f[8];
 };
 // Per-call values the FP32C packers need (see ctx_final_kernel and
 // tile_origin_kernel).

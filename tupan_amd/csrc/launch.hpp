@@ -73,6 +73,7 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     js.f[k] = jp[k];
   }
   Out out{};
+  static_assert(Pol::kOut <= 8, "at most 8 outputs");
   for (int k = 0; k < Pol::kOut; ++k) {
     if (!op[k]) return set_error(TUPAN_ERR_ARG, "NULL output field %d", k);
     out.f[k] = op[k];

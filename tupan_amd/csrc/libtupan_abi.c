@@ -85,7 +85,7 @@ static void forward(const char *who, int kind, mode_fn fn, size_t ni,
                     const double *inv, double dt, int flag)
 {
     const double *iw[14] = {0}, *jw[14] = {0};
-    double *ow[6] = {0};
+    double *ow[8] = {0};
     int rc = TUPAN_OK, bad = 0;
     if (ni == 0) return;
     for (int k = 0; k < nf && !bad; ++k) bad |= WIDEN(iw[k], ih[k], ni);
@@ -107,6 +107,8 @@ static void forward(const char *who, int kind, mode_fn fn, size_t ni,
         case 1: rc = tupan_tstep_host(ni, iw, nj, jw, eta, ow); break;
         case 2: rc = tupan_pnacc_host(ni, iw, nj, jw, order, inv, ow); break;
         case 3: rc = tupan_sakura_host(ni, iw, nj, jw, dt, flag, ow); break;
+        case 4: rc = tupan_nreg_x_host(ni, iw, nj, jw, dt, ow); break;
+        case 5: rc = tupan_nreg_v_host(ni, iw, nj, jw, dt, ow); break;
         }
     }
     for (int k = 0; k < nout; ++k) {
@@ -226,4 +228,75 @@ EXPORT void sakura_kernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
     REAL *o[6] = {idrx, idry, idrz, idvx, idvy, idvz};
     forward("sakura_kernel", 3, NULL, ni, i8, nj, j8, 8, o, 6, 0, 0, NULL,
             (double)dt, (int)flag);
+}
+
+/* libtupan.h:152-179 */
+EXPORT void nreg_Xkernel(const UINT ni, R im, R irx, R iry, R irz, R ie2,
+                         R ivx, R ivy, R ivz, const UINT nj, R jm, R jrx,
+                         R jry, R jrz, R je2, R jvx, R jvy, R jvz,
+                         const REAL dt, REAL *restrict idrx,
+                         REAL *restrict idry, REAL *restrict idrz,
+                         REAL *restrict iax, REAL *restrict iay,
+                         REAL *restrict iaz, REAL *restrict iu)
+{
+    const REAL *i8[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const REAL *j8[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    REAL *o[7] = {idrx, idry, idrz, iax, iay, iaz, iu};
+    forward("nreg_Xkernel", 4, NULL, ni, i8, nj, j8, 8, o, 7, 0, 0, NULL,
+            (double)dt, 0);
+}
+
+/* libtupan.h:181-201 */
+EXPORT void nreg_Vkernel(const UINT ni, R im, R ivx, R ivy, R ivz, R iax,
+                         R iay, R iaz, const UINT nj, R jm, R jvx, R jvy,
+                         R jvz, R jax, R jay, R jaz, const REAL dt,
+                         REAL *restrict idvx, REAL *restrict idvy,
+                         REAL *restrict idvz, REAL *restrict ik)
+{
+    const REAL *i7[7] = {im, ivx, ivy, ivz, iax, iay, iaz};
+    const REAL *j7[7] = {jm, jvx, jvy, jvz, jax, jay, jaz};
+    REAL *o[4] = {idvx, idvy, idvz, ik};
+    forward("nreg_Vkernel", 5, NULL, ni, i7, nj, j7, 7, o, 4, 0, 0, NULL,
+            (double)dt, 0);
+}
+
+/* libtupan.h:231-246: particles 0 and 1, outputs may alias the inputs
+ * (the reference's Kepler extension updates them in place) -- all inputs
+ * are read before any output is written. */
+EXPORT void kepler_solver_kernel(const REAL *im, const REAL *irx,
+                                 const REAL *iry, const REAL *irz,
+                                 const REAL *ie2, const REAL *ivx,
+                                 const REAL *ivy, const REAL *ivz,
+                                 const REAL dt, REAL *ir1x, REAL *ir1y,
+                                 REAL *ir1z, REAL *iv1x, REAL *iv1y,
+                                 REAL *iv1z)
+{
+    const REAL *in[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    REAL *out[6] = {ir1x, ir1y, ir1z, iv1x, iv1y, iv1z};
+    double buf[8][2], res[6][2];
+    const double *ip[8];
+    double *op[6];
+    for (int k = 0; k < 8; ++k) {
+        buf[k][0] = (double)in[k][0];
+        buf[k][1] = (double)in[k][1];
+        ip[k] = buf[k];
+    }
+    for (int k = 0; k < 6; ++k) op[k] = res[k];
+    int rc = tupan_kepler_host(ip, (double)dt, op);
+    g_status = rc;
+    if (rc != TUPAN_OK) {
+        static int printed = 0;
+        if (!printed) {
+            fprintf(stderr, "kepler_solver_kernel: %s\n", tupan_last_error());
+            printed = 1;
+        }
+        if (rc != TUPAN_ERR_NOT_CONVERGED) {
+            for (int k = 0; k < 6; ++k) out[k][0] = out[k][1] = (REAL)NAN;
+            return;
+        }
+    }
+    for (int k = 0; k < 6; ++k) {
+        out[k][0] = (REAL)res[k][0];
+        out[k][1] = (REAL)res[k][1];
+    }
 }

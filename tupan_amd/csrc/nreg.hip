@@ -1,0 +1,254 @@
+// nreg.hip -- the NREG pair sums and the two-body Kepler kernel, FP64.
+//
+//   NREG X  tupan/lib/src/nreg_kernels_common.h:7-58 (driver
+//           nreg_kernels.c:5-61): sums at the drifted separation
+//           r_ij + v_ij dt of m_j r, the softened acceleration and
+//           m_j / s; the last is finished as u_i = m_i * sum.
+//   NREG V  nreg_kernels_common.h:61-99 (driver nreg_kernels.c:64-115):
+//           sums at the kicked relative velocity w = v_ij + a_ij dt of
+//           m_j w and m_j |w|^2, the last finished as k_i = m_i * sum.
+//   Kepler  kepler_solver_kernel_common.h:8-79 (driver
+//           kepler_solver_kernel.c:5-49): particles 0 and 1 advanced by dt
+//           along their softened Kepler orbit about the freely drifting
+//           centre of mass (twobody.hpp's kepler_drift).
+#include "launch.hpp"
+#include "twobody.hpp"
+
+namespace tupan {
+
+struct NregParams {
+  double dt;
+  const double *im;  // i-side masses, for the finishing factor
+};
+
+struct NregXF64 {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 8, kMaxIpl = 2, kPrefIpl = 1, kOut = 7;
+  using Params = NregParams;
+  struct JRec { double m, x, y, z, e2, vx, vy, vz; };
+  using I = JRec;
+  using A64 = F64Acc<7>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
+            s.f[E2][j], s.f[VX][j], s.f[VY][j], s.f[VZ][j]};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return pack(s, i, 0.0);
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &prm) {
+    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
+    rx += (q.vx - r.vx) * prm.dt;
+    ry += (q.vy - r.vy) * prm.dt;
+    rz += (q.vz - r.vz) * prm.dt;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = r2 > 0.0 ? y : 0.0;
+    double k = r.m * y * (y * y);
+    a.v[0] += r.m * rx;
+    a.v[1] += r.m * ry;
+    a.v[2] += r.m * rz;
+    a.v[3] -= k * rx;
+    a.v[4] -= k * ry;
+    a.v[5] -= k * rz;
+    a.v[6] += r.m * y;
+  }
+  __device__ static void combine(double *v, const double *w) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) v[k] += w[k];
+  }
+  __device__ static void finish(const double *v, Out out, int i,
+                                const Params &prm) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.f[k][i] = v[k];
+    out.f[6][i] = prm.im[i] * v[6];
+  }
+};
+
+// Fields (reference order, libtupan.h:181-197): m, vx, vy, vz, ax, ay, az.
+struct NregVF64 {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 7, kMaxIpl = 2, kPrefIpl = 1, kOut = 4;
+  using Params = NregParams;
+  struct JRec { double m, vx, vy, vz, ax, ay, az, pad; };
+  using I = JRec;
+  using A64 = F64Acc<4>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[0][j], s.f[1][j], s.f[2][j], s.f[3][j],
+            s.f[4][j], s.f[5][j], s.f[6][j], 0.0};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return pack(s, i, 0.0);
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &prm) {
+    double wx = (q.vx - r.vx) + (q.ax - r.ax) * prm.dt;
+    double wy = (q.vy - r.vy) + (q.ay - r.ay) * prm.dt;
+    double wz = (q.vz - r.vz) + (q.az - r.az) * prm.dt;
+    double w2 = wx * wx + wy * wy + wz * wz;
+    a.v[0] += r.m * wx;
+    a.v[1] += r.m * wy;
+    a.v[2] += r.m * wz;
+    a.v[3] += r.m * w2;
+  }
+  __device__ static void combine(double *v, const double *w) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] += w[k];
+  }
+  __device__ static void finish(const double *v, Out out, int i,
+                                const Params &prm) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) out.f[k][i] = v[k];
+    out.f[3][i] = prm.im[i] * v[3];
+  }
+};
+
+// One thread: reads particles 0 and 1 completely before writing, so the
+// outputs may alias the inputs (the reference updates ips in place,
+// tupan/lib/extensions.py:642-646).
+__global__ void kepler_kernel(Soa s, double dt, double *r1x, double *r1y,
+                              double *r1z, double *v1x, double *v1y,
+                              double *v1z, int budget, int *status) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const double m0 = s.f[M][0], m1 = s.f[M][1];
+  const double x0[3] = {s.f[RX][0], s.f[RY][0], s.f[RZ][0]};
+  const double x1[3] = {s.f[RX][1], s.f[RY][1], s.f[RZ][1]};
+  const double u0[3] = {s.f[VX][0], s.f[VY][0], s.f[VZ][0]};
+  const double u1[3] = {s.f[VX][1], s.f[VY][1], s.f[VZ][1]};
+  const double e2 = s.f[E2][0] + s.f[E2][1];
+  const double m = m0 + m1;
+  const double imu = m0 / m, jmu = m1 / m;
+  double st[6], rcm[3], vcm[3];
+  for (int k = 0; k < 3; ++k) {
+    st[k] = x0[k] - x1[k];
+    st[3 + k] = u0[k] - u1[k];
+    rcm[k] = imu * x0[k] + jmu * x1[k];
+    vcm[k] = imu * u0[k] + jmu * u1[k];
+    rcm[k] += vcm[k] * dt;
+  }
+  int left = budget;
+  if (!kepler_drift(dt, m, e2, st, left) && status) atomicOr(status, 1);
+  double *out_r[3] = {r1x, r1y, r1z}, *out_v[3] = {v1x, v1y, v1z};
+  for (int k = 0; k < 3; ++k) {
+    out_r[k][0] = rcm[k] + jmu * st[k];
+    out_v[k][0] = vcm[k] + jmu * st[3 + k];
+    out_r[k][1] = rcm[k] - imu * st[k];
+    out_v[k][1] = vcm[k] - imu * st[3 + k];
+  }
+}
+
+}  // namespace tupan
+
+using namespace tupan;
+
+extern "C" {
+
+int tupan_nreg_x_dev(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, double *const *op,
+                     void *stream) {
+  if (ni && (!ip || !ip[M]))
+    return set_error(TUPAN_ERR_ARG, "NULL i mass field");
+  NregParams prm{dt, ni ? ip[M] : nullptr};
+  return run_engine<NregXF64>(TUPAN_K_NREG_X, TUPAN_MODE_FP64, ni, ip, nj,
+                              jp, op, prm,
+                              reinterpret_cast<hipStream_t>(stream));
+}
+
+int tupan_nreg_x_host(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double dt, double *const *op) {
+  return host_call(ni, ip, nj, jp, 8, op, 7,
+                   [&](const double *const *di, const double *const *dj,
+                       double *const *dout, hipStream_t st) {
+                     return tupan_nreg_x_dev(ni, di, nj, dj, dt, dout, st);
+                   });
+}
+
+int tupan_nreg_v_dev(size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, double *const *op,
+                     void *stream) {
+  if (ni && (!ip || !ip[0]))
+    return set_error(TUPAN_ERR_ARG, "NULL i mass field");
+  NregParams prm{dt, ni ? ip[0] : nullptr};
+  return run_engine<NregVF64>(TUPAN_K_NREG_V, TUPAN_MODE_FP64, ni, ip, nj,
+                              jp, op, prm,
+                              reinterpret_cast<hipStream_t>(stream));
+}
+
+int tupan_nreg_v_host(size_t ni, const double *const *ip, size_t nj,
+                      const double *const *jp, double dt, double *const *op) {
+  return host_call(ni, ip, nj, jp, 7, op, 4,
+                   [&](const double *const *di, const double *const *dj,
+                       double *const *dout, hipStream_t st) {
+                     return tupan_nreg_v_dev(ni, di, nj, dj, dt, dout, st);
+                   });
+}
+
+int tupan_kepler_dev(const double *const *ip, double dt, double *const *op,
+                     int *d_status, void *stream) {
+  if (!ip || !op) return set_error(TUPAN_ERR_ARG, "NULL field array");
+  Soa s{};
+  for (int k = 0; k < 8; ++k) {
+    if (!ip[k]) return set_error(TUPAN_ERR_ARG, "NULL input field %d", k);
+    s.f[k] = ip[k];
+  }
+  for (int k = 0; k < 6; ++k)
+    if (!op[k]) return set_error(TUPAN_ERR_ARG, "NULL output field %d", k);
+  hipLaunchKernelGGL(kepler_kernel, dim3(1), dim3(64), 0,
+                     reinterpret_cast<hipStream_t>(stream), s, dt, op[0],
+                     op[1], op[2], op[3], op[4], op[5], sakura_budget(),
+                     d_status);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TUPAN_OK : hip_fail(e, "kepler launch");
+}
+
+int tupan_kepler_host(const double *const *ip, double dt, double *const *op) {
+  // two particles: stage them in one small device buffer
+  double h[8 * 2], *d = nullptr;
+  for (int k = 0; k < 8; ++k) {
+    if (!ip || !ip[k]) return set_error(TUPAN_ERR_ARG, "NULL input field");
+    h[2 * k] = ip[k][0];
+    h[2 * k + 1] = ip[k][1];
+  }
+  if (!op) return set_error(TUPAN_ERR_ARG, "NULL output field array");
+  int rc = device_alloc((void **)&d, sizeof(double) * (16 + 12) + 64, nullptr);
+  if (rc) return rc;
+  int *status = reinterpret_cast<int *>(d + 28);
+  hipError_t e = hipMemcpy(d, h, sizeof h, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(status, 0, sizeof(int));
+  if (e != hipSuccess) {
+    device_free(d, nullptr);
+    return hip_fail(e, "kepler staging");
+  }
+  const double *dip[8];
+  double *dop[6];
+  for (int k = 0; k < 8; ++k) dip[k] = d + 2 * k;
+  for (int k = 0; k < 6; ++k) dop[k] = d + 16 + 2 * k;
+  rc = tupan_kepler_dev(dip, dt, dop, status, nullptr);
+  double o[12];
+  int hs = 0;
+  if (!rc) {
+    e = hipMemcpy(o, d + 16, sizeof o, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      e = hipMemcpy(&hs, status, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "kepler copy-back");
+  }
+  device_free(d, nullptr);
+  (void)hipStreamSynchronize(nullptr);
+  if (rc) return rc;
+  for (int k = 0; k < 6; ++k) {
+    if (!op[k]) return set_error(TUPAN_ERR_ARG, "NULL output field %d", k);
+    op[k][0] = o[2 * k];
+    op[k][1] = o[2 * k + 1];
+  }
+  if (hs)
+    return set_error(TUPAN_ERR_NOT_CONVERGED,
+                     "the Kepler solve exhausted its budget of %d solves",
+                     sakura_budget());
+  return TUPAN_OK;
+}
+
+}  // extern "C"

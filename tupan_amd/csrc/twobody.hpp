@@ -79,7 +79,7 @@ TUPAN_HD inline void kep_eval(double s, const KepEq &k, double *f,
 
 // Laguerre-Conway iteration, order 5 (universal_kepler_solver.h:225-254):
 // 0 = converged, -1 zero denominator, -2 iteration cap, -3 sign flip.
-TUPAN_HD int laguerre(double s0, double *s, const KepEq &k) {
+TUPAN_HD inline int laguerre(double s0, double *s, const KepEq &k) {
   double x = s0, delta;
   int it = 0;
   do {
@@ -101,7 +101,7 @@ TUPAN_HD int laguerre(double s0, double *s, const KepEq &k) {
 
 // One Kepler step of the softened problem (universal_kepler_solver.h:
 // 312-463); st = (r, v) in and out, untouched on failure.
-TUPAN_HD int kepler_once(double dt0, double m, double e2, double *st) {
+TUPAN_HD inline int kepler_once(double dt0, double m, double e2, double *st) {
   double rx = st[0], ry = st[1], rz = st[2];
   double vx = st[3], vy = st[4], vz = st[5];
   double r2 = rx * rx + ry * ry + rz * rz;
@@ -152,7 +152,7 @@ TUPAN_HD int kepler_once(double dt0, double m, double e2, double *st) {
 
 // n = 1, 2, 4, ... equal sub-steps until all converge (:466-507).
 // Returns false when the budget ran out (out = last attempt's start).
-TUPAN_HD bool kepler_substeps(double dt, double m, double e2,
+TUPAN_HD inline bool kepler_substeps(double dt, double m, double e2,
                                 const double *in, double *out, int &budget) {
   for (long n = 1;; n *= 2) {
     double st[6] = {in[0], in[1], in[2], in[3], in[4], in[5]};
@@ -183,7 +183,7 @@ TUPAN_HD inline double kep_energy(const double *st, double m,
 
 // Softened Kepler drift with the energy-conservation sub-stepping
 // (universal_kepler_solver.h:510-614).
-TUPAN_HD bool kepler_drift(double dt, double m, double e2, double *st,
+TUPAN_HD inline bool kepler_drift(double dt, double m, double e2, double *st,
                              int &budget) {
   double in[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
   double cur[6];
@@ -242,7 +242,7 @@ TUPAN_HD inline void pair_leapfrog(double dt, double m, double e2,
   st[5] = vz;
 }
 
-TUPAN_HD bool pair_twobody(double dt, double m, double e2, double *st,
+TUPAN_HD inline bool pair_twobody(double dt, double m, double e2, double *st,
                              int &budget) {
   double r2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
   double v2 = st[3] * st[3] + st[4] * st[4] + st[5] * st[5];
@@ -263,7 +263,7 @@ TUPAN_HD inline void free_drift(double *st, double t) {
 // flag: -1 drift then K(dt); +1 K(dt) then drift; -2 half drift, K(dt),
 // half drift; +2 K(dt/2), drift, K(dt/2); otherwise identity
 // (sakura_kernel_common.h:151-183).  "drift" moves by -v dt.
-TUPAN_HD bool pair_evolve(double dt, int flag, double m, double e2,
+TUPAN_HD inline bool pair_evolve(double dt, int flag, double m, double e2,
                             double *st, int budget = kBudget) {
   bool ok = true;
   switch (flag) {

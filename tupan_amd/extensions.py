@@ -33,7 +33,8 @@ from . import _lib, spec
 
 __all__ = ["Phi", "phi", "Acc", "acc", "AccJerk", "acc_jerk",
            "SnapCrackle", "snap_crackle", "Tstep", "tstep",
-           "PNAcc", "pnacc", "Sakura", "sakura", "Clight", "clight",
+           "PNAcc", "pnacc", "Sakura", "sakura", "NREG_X", "nreg_x",
+           "NREG_V", "nreg_v", "Kepler", "kepler", "Clight", "clight",
            "get_kernel", "register_backend"]
 
 logger = logging.getLogger(__name__)
@@ -111,6 +112,8 @@ class HIPKernel(object):
         return arrays
 
     def run(self):
+        if self.key == "kepler":
+            return self._run_kepler()
         k = spec.KERNELS[self.key]
         nf = len(k.fields)
         a = self.args
@@ -135,6 +138,18 @@ class HIPKernel(object):
         if wide:
             for o, w in zip(outs, ow):
                 o[:ni] = w
+
+
+    def _run_kepler(self):
+        # (8 input arrays, dt, 6 outputs) -- libtupan.h:231-246; particles
+        # 0 and 1, outputs may be the input arrays themselves
+        a = self.args
+        ins = [np.ascontiguousarray(x[:2], np.float64) for x in a[:8]]
+        res = [np.empty(2, np.float64) for _ in range(6)]
+        _lib.call_kepler_host([x.ctypes.data for x in ins], float(a[8]),
+                              [o.ctypes.data for o in res])
+        for o, r in zip(a[9:15], res):
+            o[:2] = r
 
 
 _BACKENDS = {"HIP": HIPKernel}
@@ -297,6 +312,47 @@ class Sakura(_PairExtension):
         _PairExtension.set_args(self, ips, jps, dt, flag)
 
 
+class NREG_X(_PairExtension):
+    """NREG position-stage sums at the drifted separation r + v dt:
+    ``mrx, mry, mrz`` (sum_j m_j r), ``ax, ay, az`` and ``u``
+    (extensions.py:510-561)."""
+    symbol = "nreg_Xkernel"
+    nscalars = ("c_real",)
+
+    def set_args(self, ips, jps, dt):
+        _PairExtension.set_args(self, ips, jps, dt)
+
+
+class NREG_V(_PairExtension):
+    """NREG velocity-stage sums at the kicked velocity v + a dt: ``mvx, mvy,
+    mvz`` and ``mk`` (extensions.py:564-617)."""
+    symbol = "nreg_Vkernel"
+    nscalars = ("c_real",)
+
+    def set_args(self, ips, jps, dt):
+        _PairExtension.set_args(self, ips, jps, dt)
+
+
+class Kepler(AbstractExtension):
+    """Two-body Kepler solver: advances particles 0 and 1 of ``ips`` by dt
+    in place (extensions.py:620-651)."""
+
+    def __init__(self, backend, prec, mode=None):
+        if backend == "CL":       # as the reference: C is fast enough
+            backend = "HIP"
+        self.kernel = get_kernel("kepler_solver_kernel", backend, prec)
+        cty = self.kernel.cty
+        self.argtypes = (cty.c_real_p,) * 8 + (cty.c_real,)
+        self.restypes = (cty.c_real_p,) * 6
+
+    def set_args(self, ips, jps, dt):
+        self._inargs = tuple(getattr(ips, f) for f in spec.KEPLER_FIELDS) + (dt,)
+        self._outargs = tuple(getattr(ips, f) for f in spec.KEPLER_OUTS)
+        self.inargs = prepare_args(self._inargs, self.argtypes)
+        self.outargs = prepare_args(self._outargs, self.restypes)
+        self.kernel.set_args(self.inargs + self.outargs)
+
+
 backend = os.environ.get("TUPAN_BACKEND", "HIP")
 _prec = "float32" if "--use_sp" in sys.argv else "float64"
 
@@ -308,3 +364,6 @@ snap_crackle = SnapCrackle(backend, _prec)
 tstep = Tstep(backend, _prec)
 pnacc = PNAcc(backend, _prec)
 sakura = Sakura(backend, _prec)
+nreg_x = NREG_X(backend, _prec)
+nreg_v = NREG_V(backend, _prec)
+kepler = Kepler(backend, _prec)

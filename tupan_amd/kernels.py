@@ -9,6 +9,8 @@ Extra scalar arguments per kernel (as in tupan/lib/extensions.py):
   tstep:  eta
   pnacc:  order, (inv1, ..., inv7)
   sakura: dt, flag
+  nreg_x, nreg_v: dt
+``kepler(ps, dt)`` runs the two-particle Kepler solver.
 """
 import numpy as np
 
@@ -45,3 +47,13 @@ def compute(name, ips, jps, *args, mode="fp64", out=None):
     optr = [o.ctypes.data for o in out]
     _lib.call_host(name, mode, ni, iptr, nj, jptr, optr, args)
     return out
+
+
+def kepler(ps, dt):
+    """kepler_solver_kernel (libtupan.h:231-246) on particles 0 and 1 of
+    ``ps``: returns the new (rx, ry, rz, vx, vy, vz), each of length 2."""
+    ins = [_lib.as_f64(_field(ps, f))[:2].copy() for f in spec.KEPLER_FIELDS]
+    outs = tuple(np.empty(2, dtype=np.float64) for _ in spec.KEPLER_OUTS)
+    _lib.call_kepler_host([a.ctypes.data for a in ins], dt,
+                          [o.ctypes.data for o in outs])
+    return outs

@@ -3,7 +3,7 @@
 Field order is the reference's argument order (tupan/lib/src/libtupan.h);
 output attribute names are the ones tupan/lib/extensions.py attaches to the
 i-particles (Phi :132, Acc :198, AccJerk :279-280, SnapCrackle :337-338,
-Tstep :382, PNAcc :434, Sakura :501-502).
+Tstep :382, PNAcc :434, Sakura :501-502, NREG_X :555-557, NREG_V :609-610).
 
 ``flops`` is the reference's own per-interaction count from the comments
 of each *_kernel_common.h (phi :32, acc :38, acc_jerk :56, snap_crackle :95,
@@ -28,6 +28,10 @@ KERNELS = {
     "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), False, None),
     "sakura": Kernel("sakura", 6, F8,
                      ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
+    "nreg_x": Kernel("nreg_x", 9, F8,
+                     ("mrx", "mry", "mrz", "ax", "ay", "az", "u"), False, 37),
+    "nreg_v": Kernel("nreg_v", 10, ("mass", "vx", "vy", "vz", "ax", "ay", "az"),
+                     ("mvx", "mvy", "mvz", "mk"), False, 25),
     # pair-selected variants for hybrid schemes (no reference counterpart):
     # Acc over the pairs with r >= rcut, Sakura over the pairs with r < rcut
     "acc_far": Kernel("acc_far", 1, F5, ("ax", "ay", "az"), False, 20),
@@ -55,4 +59,11 @@ SYMBOLS = {
     "tstep_kernel": "tstep",
     "pnacc_kernel": "pnacc",
     "sakura_kernel": "sakura",
+    "nreg_Xkernel": "nreg_x",
+    "nreg_Vkernel": "nreg_v",
+    "kepler_solver_kernel": "kepler",
 }
+
+# kepler_solver_kernel (libtupan.h:231-246): two particles, updated in place
+KEPLER_FIELDS = F8
+KEPLER_OUTS = ("rx", "ry", "rz", "vx", "vy", "vz")
