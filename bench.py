@@ -149,7 +149,8 @@ def cpu_baseline(ps, n, target_s):
 
 
 ENS_CANDIDATES = [(1, 256, 8), (1, 512, 8), (1, 1024, 8), (2, 512, 8),
-                  (1, 512, 4), (1, 512, 16)]
+                  (1, 512, 4), (1, 512, 16), (1, 1024, 16), (2, 512, 16),
+                  (1, 512, 32)]
 
 
 def bench_ensemble(args, torch):
