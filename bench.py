@@ -17,6 +17,10 @@ Reported:
   accuracy    per-particle |a - a64|/|a64| and |j - j64|/|j64| (max, rms)
               against the FP64 kernel on the same inputs
   fp64        throughput of the FP64 kernel on the same workload
+  fp64_ieee   throughput of the reference-arithmetic mode (the reference C's
+              operations, order and rounding on the GPU), whose outputs on
+              the cpu_baseline's sampled rows must equal the reference C's
+              bit for bit
   cpu_baseline  the reference's C AccJerk (oracle/_ref, or our bit-identical
               C restatement when _ref is absent), single-threaded, on a
               sample of i-rows x all N j-particles
@@ -44,7 +48,8 @@ def parse():
     p.add_argument("--steps", type=int, default=30)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--n", type=int, default=65536)
-    p.add_argument("--mode", default="fp32c", choices=["fp32c", "fp64"])
+    p.add_argument("--mode", default="fp32c",
+                   choices=["fp32c", "fp64", "fp64_ieee"])
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target CPU work for the cpu_baseline sample")
     p.add_argument("--no-cpu", action="store_true")
@@ -119,7 +124,10 @@ def time_steps(fn, steps, warmup, torch):
     return elapsed, k_ms / max(k_n, 1), k_n
 
 
-def cpu_baseline(ps, n, target_s):
+def cpu_baseline(ps, n, target_s, gpu_ieee=None):
+    """Reference C AccJerk on a bounded sample of i-rows.  gpu_ieee: host
+    copies of the GPU's fp64_ieee outputs for all n rows; the sample's
+    outputs are compared with them bit for bit."""
     import oracle
     kind = "reference" if oracle.available("float64", "reference") else "port"
     o = oracle.Oracle("float64", kind)
@@ -131,21 +139,27 @@ def cpu_baseline(ps, n, target_s):
         idx = np.sort(rng.choice(n, rows, replace=False))
         idd = {f: v[idx] for f, v in jd.items()}
         t0 = time.perf_counter()
-        o.call("acc_jerk_kernel", idd, jd)
-        return time.perf_counter() - t0
+        res = o.call("acc_jerk_kernel", idd, jd)
+        return time.perf_counter() - t0, idx, res
 
     probe_rows = 64
-    dt = run(probe_rows)
+    dt, _, _ = run(probe_rows)
     rows = int(min(n, max(probe_rows, probe_rows * target_s / max(dt, 1e-6))))
-    dt = run(rows)
+    dt, idx, res = run(rows)
     rate = rows * n / dt
-    return {"value": rate / 1e9, "unit": "Ginteractions/s", "cores": 1,
+    extra = {}
+    if gpu_ieee is not None:
+        same = all(np.array_equal(r.view(np.uint64), g[idx].view(np.uint64))
+                   for r, g in zip(res, gpu_ieee))
+        extra["fp64_ieee_bitwise_equal"] = bool(same)
+    return dict(extra, **{"value": rate / 1e9, "unit": "Ginteractions/s",
+                          "cores": 1,
             "kind": kind,
             "sample": "AccJerk FP64, %d sampled i-rows x all %d j (%.1f s, "
                       "1 thread, %s)" % (rows, n, dt,
                                          "reference C tupan/lib/src compiled "
                                          "-O3 -std=c99" if kind == "reference"
-                                         else "bit-identical C restatement")}
+                                         else "bit-identical C restatement")})
 
 
 ENS_CANDIDATES = [(1, 256, 8), (1, 512, 8), (1, 1024, 8), (2, 512, 8),
@@ -463,9 +477,22 @@ def main():
                           "unit": "Ginteractions/s", "kernel_ms": k64}
         result["accuracy"] = rel_err(outs, ref)
         result["accuracy"]["vs"] = "FP64 GPU kernel, same inputs"
+    gpu_ieee = None
+    if args.mode == "fp32c" and not args.no_fp64:
+        ieee = tuple(torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(6))
+
+        def step_ieee():
+            device.compute("acc_jerk", d, d, mode="fp64_ieee", out=ieee)
+
+        print("[bench] fp64_ieee timing", file=sys.stderr, flush=True)
+        ei, ki, _ = time_steps(step_ieee, 3, 1, torch)
+        result["fp64_ieee"] = {"value": pairs * 3 / ei / 1e9,
+                               "unit": "Ginteractions/s", "kernel_ms": ki}
+        gpu_ieee = [t.cpu().numpy() for t in ieee]
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(ps, n, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(ps, n, args.cpu_seconds,
+                                              gpu_ieee)
     print(json.dumps(result))
 
 

@@ -25,11 +25,16 @@
  * accumulated) for i < ni; i and j arrays may alias (ips is jps); ni or
  * nj = 0 is valid (nj = 0 writes the empty sums).
  *
- * mode: TUPAN_MODE_FP64  -- FP64 arithmetic (oracle-grade)
- *       TUPAN_MODE_FP32C -- FP32 pair arithmetic on FP64-exact position
- *                           differences with FP64 block accumulation
- *                           (Phi, Acc, AccJerk, SnapCrackle; Tstep, PNAcc
- *                           and Sakura always run FP64 and ignore it).
+ * mode: TUPAN_MODE_FP64      -- FP64 arithmetic (oracle-grade)
+ *       TUPAN_MODE_FP32C     -- FP32 pair arithmetic on FP64-exact position
+ *                               differences with FP64 block accumulation
+ *       TUPAN_MODE_FP64_IEEE -- the reference C's arithmetic bit for bit:
+ *                               its operations in its order, no FMA
+ *                               contraction, correctly rounded division and
+ *                               sqrt, j summed in ascending order (one j
+ *                               split); results equal the reference's bits
+ *       (modes apply to Phi, Acc, AccJerk, SnapCrackle; Tstep, PNAcc, Sakura
+ *       and NREG always run FP64 and take no mode).
  *
  * Every function returns 0 on success or a TUPAN_ERR_* code; the text of the
  * last error on the calling thread is tupan_last_error().
@@ -50,6 +55,8 @@ extern "C" {
 
 #define TUPAN_MODE_FP64 0
 #define TUPAN_MODE_FP32C 1
+#define TUPAN_MODE_FP64_IEEE 2
+#define TUPAN_MODE_COUNT 3
 
 /* kernel ids for tupan_set_launch */
 #define TUPAN_K_PHI 0
@@ -74,7 +81,8 @@ int tupan_set_default_mode(int mode);
 int tupan_get_default_mode(void);
 /* Launch shape of the pair engine (autotuner hook): ipl = i-particles per
  * lane (1, 2 or 4; 0 = automatic), splits = number of j-range splits
- * (0 = automatic). */
+ * (0 = automatic; TUPAN_MODE_FP64_IEEE always runs one split and accepts
+ * only 0 or 1). */
 int tupan_set_launch(int kernel_id, int mode, int ipl, int splits);
 /* Live timing of the pair-engine kernel: while on, every engine launch is
  * bracketed by HIP events on its stream.  tupan_take_kernel_times waits for

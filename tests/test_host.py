@@ -68,6 +68,13 @@ def test_version_and_mode_calls_need_no_device():
     assert L.tupan_set_default_mode(7) == _lib.ERR_ARG
     assert L.tupan_set_launch(2, 1, 3, 0) == _lib.ERR_ARG
     assert L.tupan_set_launch(2, 1, 0, 0) == _lib.OK
+    # the reference-arithmetic mode keeps the reference's single j sum
+    assert L.tupan_set_launch(2, _lib.MODE_FP64_IEEE, 2, 1) == _lib.OK
+    assert L.tupan_set_launch(2, _lib.MODE_FP64_IEEE, 0, 4) == _lib.ERR_ARG
+    assert L.tupan_set_launch(2, _lib.MODE_FP64_IEEE, 0, 0) == _lib.OK
+    assert L.tupan_set_launch(2, 3, 0, 0) == _lib.ERR_ARG
+    assert L.tupan_set_default_mode(_lib.MODE_FP64_IEEE) == _lib.OK
+    assert L.tupan_set_default_mode(_lib.MODE_FP64) == _lib.OK
 
 
 def test_missing_library_fails_loudly(monkeypatch):

@@ -18,7 +18,8 @@ LIB_PATH = os.environ.get("TUPAN_HIP_LIB",
 
 MODE_FP64 = 0
 MODE_FP32C = 1
-MODES = {"fp64": MODE_FP64, "fp32c": MODE_FP32C}
+MODE_FP64_IEEE = 2
+MODES = {"fp64": MODE_FP64, "fp32c": MODE_FP32C, "fp64_ieee": MODE_FP64_IEEE}
 
 OK = 0
 ERR_HIP = 1

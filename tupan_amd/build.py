@@ -23,7 +23,8 @@ OBJDIR = os.path.join(LIBDIR, "obj")
 ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["runtime.hip", "gravity.hip", "pnacc.hip", "sakura.hip",
-               "integrate.hip", "adjoint.hip", "ensemble.hip", "nreg.hip"]
+               "integrate.hip", "adjoint.hip", "ensemble.hip", "nreg.hip",
+               "ieee.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TUPAN_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",

@@ -440,6 +440,8 @@ static int dispatch(int kid, int mode, size_t ni, const double *const *ip,
     return run_engine<P64>(kid, mode, ni, ip, nj, jp, op, {}, st);
   if (mode == TUPAN_MODE_FP32C)
     return run_engine<P32c>(kid, mode, ni, ip, nj, jp, op, {}, st);
+  if (mode == TUPAN_MODE_FP64_IEEE)
+    return run_ieee(kid, ni, ip, nj, jp, op, st);
   return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
 }
 

@@ -27,7 +27,7 @@ int hip_fail(hipError_t e, const char *what) {
 
 // ------------------------------------------------------ launch shapes
 static std::mutex g_mu;
-static LaunchShape g_override[TUPAN_K_COUNT][2];  // [kernel][mode], zero = automatic
+static LaunchShape g_override[TUPAN_K_COUNT][TUPAN_MODE_COUNT];  // zero = automatic
 static int g_default_mode = TUPAN_MODE_FP64;
 static double g_near_c = 2.0;          // near-field radius, see engine.hpp
 static int g_sakura_budget = 1 << 20;  // Kepler solves per Sakura pair
@@ -66,7 +66,8 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
   LaunchShape o{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (kid >= 0 && kid < TUPAN_K_COUNT && (mode == 0 || mode == 1))
+    if (kid >= 0 && kid < TUPAN_K_COUNT && mode >= 0 &&
+        mode < TUPAN_MODE_COUNT)
       o = g_override[kid][mode];
   }
   const long target = 64L * simd_count();
@@ -79,6 +80,8 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
   if (S > max_split && o.splits <= 0) S = max_split;
   if (S < 1) S = 1;
   if (nj > 0 && S > (long)nj) S = (long)nj;
+  // the reference's summation order: every j into one accumulator
+  if (mode == TUPAN_MODE_FP64_IEEE) S = 1;
   return LaunchShape{P, (int)S};
 }
 
@@ -200,7 +203,7 @@ int tupan_device_count(int *count) {
 }
 
 int tupan_set_default_mode(int mode) {
-  if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
+  if (mode < 0 || mode >= TUPAN_MODE_COUNT)
     return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
   std::lock_guard<std::mutex> lk(g_mu);
   g_default_mode = mode;
@@ -259,11 +262,13 @@ int tupan_take_kernel_times(double *total_ms, int *count) {
 int tupan_set_launch(int kernel_id, int mode, int ipl, int splits) {
   if (kernel_id < 0 || kernel_id >= TUPAN_K_COUNT)
     return set_error(TUPAN_ERR_ARG, "bad kernel id %d", kernel_id);
-  if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
+  if (mode < 0 || mode >= TUPAN_MODE_COUNT)
     return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
   if (ipl != 0 && ipl != 1 && ipl != 2 && ipl != 4)
     return set_error(TUPAN_ERR_ARG, "ipl must be 0, 1, 2 or 4");
   if (splits < 0) return set_error(TUPAN_ERR_ARG, "splits must be >= 0");
+  if (mode == TUPAN_MODE_FP64_IEEE && splits > 1)
+    return set_error(TUPAN_ERR_ARG, "the IEEE mode runs one j split");
   std::lock_guard<std::mutex> lk(g_mu);
   g_override[kernel_id][mode] = LaunchShape{ipl, splits};
   return TUPAN_OK;
