@@ -81,8 +81,9 @@ int tupan_set_default_mode(int mode);
 int tupan_get_default_mode(void);
 /* Launch shape of the pair engine (autotuner hook): ipl = i-particles per
  * lane (1, 2 or 4; 0 = automatic), splits = number of j-range splits
- * (0 = automatic; TUPAN_MODE_FP64_IEEE always runs one split and accepts
- * only 0 or 1). */
+ * (0 = automatic).  TUPAN_MODE_FP64_IEEE runs its own engine (one
+ * i-particle per lane, one ascending j sweep): it ignores ipl and accepts
+ * splits 0 or 1 only. */
 int tupan_set_launch(int kernel_id, int mode, int ipl, int splits);
 /* Live timing of the pair-engine kernel: while on, every engine launch is
  * bracketed by HIP events on its stream.  tupan_take_kernel_times waits for

@@ -4,21 +4,31 @@
 // kernels evaluate them: the same operations in the same order, one IEEE
 // operation at a time (no FMA contraction), the softened inverse square as
 // a correctly rounded division 1 / (r^2 + e2) and its square root as a
-// correctly rounded sqrt (smoothing.h:382-414), and every j summed in
-// ascending order into one accumulator per i (the mode runs with a single
-// j split).  On the same inputs the results are the reference C's bits
-// (tupan/lib/src/*_kernel_common.h, compiled as ISO C99 without
-// contraction).  This is the reproducibility mode, not the fast one: the
-// division and square root cost ~30 FP64 instructions per pair and a single
-// split leaves the chip underfilled below a few hundred thousand particles.
+// correctly rounded sqrt (smoothing.h:382-414), and every j subtracted from
+// one accumulator per i in ascending order.  On the same inputs the results
+// are the reference C's bits (tupan/lib/src/*_kernel_common.h, compiled as
+// ISO C99 without contraction).
+//
+// Engine.  The sequential j sum leaves one independent chain per (i,
+// output), so at N = 65536 the chip holds one wave per SIMD; the engine
+// therefore feeds that wave without waits: the block's four waves share j
+// tiles of 64 packed records that are double-buffered through LDS (the next
+// tile's global loads are in flight while the current one is summed), and
+// each wave computes the pair terms of U consecutive j's independently --
+// the division and square-root chains of U pairs overlap -- before
+// subtracting them from the accumulators in j order.
 #include "launch.hpp"
 
 namespace tupan {
 namespace {
 
+constexpr int kIeeeTile = 64;  // j records per LDS tile
+
+// A policy supplies JRec (a multiple of 16 bytes, read from LDS as 16-byte
+// chunks), I, pack, load_i and terms(q, r, t): the kOut per-pair terms t
+// that the reference subtracts from its sums (sum -= t).
 struct IeeeBase {
   static constexpr bool kTwoLevel = false;
-  using Params = NoParams;
 };
 
 // smoothing.h:382-390 / :405-414 -- 1/(r^2+e2), zero unless r^2 > 0
@@ -28,22 +38,18 @@ __device__ __forceinline__ double ieee_inv_r2(double r2, double e2) {
   return r2 > 0 ? inv : 0.0;
 }
 
-struct PhiIeee : IeeeBase, SumOut<1> {
-  static constexpr int kFields = 5, kMaxIpl = 2, kPrefIpl = 1;
-  struct JRec { double m, x, y, z, e2; };
+struct PhiIeee : IeeeBase {
+  static constexpr int kOut = 1, kFields = 5;
+  struct JRec { double m, x, y, z, e2, pad; };
   struct I { double x, y, z, e2; };
-  using A64 = F64Acc<1>;
   __device__ static JRec pack(const Soa &s, int j, double) {
-    return {s.f[M][j], s.f[RX][j], s.f[RY][j], s.f[RZ][j], s.f[E2][j]};
+    return {s.f[M][j], s.f[RX][j], s.f[RY][j], s.f[RZ][j], s.f[E2][j], 0.0};
   }
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
+  __device__ static I load_i(const Soa &s, int i) {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i]};
   }
-  __device__ static void zero(A64 &a) { zero_acc(a); }
-  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   // phi_kernel_common.h:7-31
-  __device__ static void pair(const I &q, const JRec &r, A64 &a,
-                              const Params &) {
+  __device__ static void terms(const I &q, const JRec &r, double *t) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -51,26 +57,22 @@ struct PhiIeee : IeeeBase, SumOut<1> {
     double e2 = q.e2 + r.e2;
     double r2 = rx * rx + ry * ry + rz * rz;
     double inv_r1 = sqrt(ieee_inv_r2(r2, e2));
-    a.v[0] -= r.m * inv_r1;
+    t[0] = r.m * inv_r1;
   }
 };
 
-struct AccIeee : IeeeBase, SumOut<3> {
-  static constexpr int kFields = 5, kMaxIpl = 2, kPrefIpl = 1;
+struct AccIeee : IeeeBase {
+  static constexpr int kOut = 3, kFields = 5;
   using JRec = PhiIeee::JRec;
   using I = PhiIeee::I;
-  using A64 = F64Acc<3>;
   __device__ static JRec pack(const Soa &s, int j, double c) {
     return PhiIeee::pack(s, j, c);
   }
-  __device__ static I load_i(const Soa &s, int i, const Params &p) {
-    return PhiIeee::load_i(s, i, p);
+  __device__ static I load_i(const Soa &s, int i) {
+    return PhiIeee::load_i(s, i);
   }
-  __device__ static void zero(A64 &a) { zero_acc(a); }
-  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   // acc_kernel_common.h:7-37
-  __device__ static void pair(const I &q, const JRec &r, A64 &a,
-                              const Params &) {
+  __device__ static void terms(const I &q, const JRec &r, double *t) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -80,30 +82,26 @@ struct AccIeee : IeeeBase, SumOut<3> {
     double inv_r2 = ieee_inv_r2(r2, e2);
     double k = inv_r2 * sqrt(inv_r2);
     k *= r.m;
-    a.v[0] -= k * rx;
-    a.v[1] -= k * ry;
-    a.v[2] -= k * rz;
+    t[0] = k * rx;
+    t[1] = k * ry;
+    t[2] = k * rz;
   }
 };
 
-struct AccJerkIeee : IeeeBase, SumOut<6> {
-  static constexpr int kFields = 8, kMaxIpl = 2, kPrefIpl = 1;
+struct AccJerkIeee : IeeeBase {
+  static constexpr int kOut = 6, kFields = 8;
   struct JRec { double m, x, y, z, e2, vx, vy, vz; };
   struct I { double x, y, z, e2, vx, vy, vz; };
-  using A64 = F64Acc<6>;
   __device__ static JRec pack(const Soa &s, int j, double) {
     return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
             s.f[E2][j], s.f[VX][j], s.f[VY][j], s.f[VZ][j]};
   }
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
+  __device__ static I load_i(const Soa &s, int i) {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i],
             s.f[VX][i], s.f[VY][i], s.f[VZ][i]};
   }
-  __device__ static void zero(A64 &a) { zero_acc(a); }
-  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   // acc_jerk_kernel_common.h:7-55
-  __device__ static void pair(const I &q, const JRec &r, A64 &a,
-                              const Params &) {
+  __device__ static void terms(const I &q, const JRec &r, double *t) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -121,35 +119,31 @@ struct AccJerkIeee : IeeeBase, SumOut<6> {
     vy -= alpha * ry;
     vz -= alpha * rz;
     k *= r.m;
-    a.v[0] -= k * rx;
-    a.v[1] -= k * ry;
-    a.v[2] -= k * rz;
-    a.v[3] -= k * vx;
-    a.v[4] -= k * vy;
-    a.v[5] -= k * vz;
+    t[0] = k * rx;
+    t[1] = k * ry;
+    t[2] = k * rz;
+    t[3] = k * vx;
+    t[4] = k * vy;
+    t[5] = k * vz;
   }
 };
 
-struct SnapCrackleIeee : IeeeBase, SumOut<6> {
-  static constexpr int kFields = 14, kMaxIpl = 1, kPrefIpl = 1;
+struct SnapCrackleIeee : IeeeBase {
+  static constexpr int kOut = 6, kFields = 14;
   struct JRec { double m, x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
   struct I { double x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
-  using A64 = F64Acc<6>;
   __device__ static JRec pack(const Soa &s, int j, double) {
     return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j], s.f[E2][j],
             s.f[VX][j], s.f[VY][j], s.f[VZ][j], s.f[AX][j], s.f[AY][j],
             s.f[AZ][j], s.f[JX][j], s.f[JY][j], s.f[JZ][j]};
   }
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
+  __device__ static I load_i(const Soa &s, int i) {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i], s.f[VX][i],
             s.f[VY][i], s.f[VZ][i], s.f[AX][i], s.f[AY][i], s.f[AZ][i],
             s.f[JX][i], s.f[JY][i], s.f[JZ][i]};
   }
-  __device__ static void zero(A64 &a) { zero_acc(a); }
-  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   // snap_crackle_kernel_common.h:7-94
-  __device__ static void pair(const I &q, const JRec &r, A64 &a,
-                              const Params &) {
+  __device__ static void terms(const I &q, const JRec &r, double *t) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -191,14 +185,141 @@ struct SnapCrackleIeee : IeeeBase, SumOut<6> {
     Jy -= alpha * Ay + beta * vy + gamma * ry;
     Jz -= alpha * Az + beta * vz + gamma * rz;
     k *= r.m;
-    a.v[0] -= k * Ax;
-    a.v[1] -= k * Ay;
-    a.v[2] -= k * Az;
-    a.v[3] -= k * Jx;
-    a.v[4] -= k * Jy;
-    a.v[5] -= k * Jz;
+    t[0] = k * Ax;
+    t[1] = k * Ay;
+    t[2] = k * Az;
+    t[3] = k * Jx;
+    t[4] = k * Jy;
+    t[5] = k * Jz;
   }
 };
+
+template <int N>
+__device__ __forceinline__ void subtract_terms(double *acc, const double *t) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int c = 0; c < N; ++c) acc[c] = acc[c] - t[c];
+}
+
+// One wave per 64 i-particles (one per lane); blocks of 4 waves share the
+// j tiles.  Every thread of the block takes part in every tile copy and
+// barrier; waves past ni skip only the sums.
+template <class Pol, int U>
+__global__ __launch_bounds__(256) void ieee_engine(
+    Soa is, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
+    Out out) {
+  using JRec = typename Pol::JRec;
+  static_assert(sizeof(JRec) % 16 == 0, "records are copied in 16 B chunks");
+  constexpr int kRecChunks = (int)(sizeof(JRec) / 16);
+  constexpr int kPer = (kIeeeTile * kRecChunks + 255) / 256;
+  __shared__ __align__(16) JRec tile[2][kIeeeTile];
+
+  const int lane = threadIdx.x & 63;
+  const int i0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  const bool active = i0 < ni;  // wave-uniform
+  const int i = min(i0 + lane, ni - 1);
+  const typename Pol::I q = Pol::load_i(is, i);
+  double acc[Pol::kOut];
+#pragma unroll
+  for (int c = 0; c < Pol::kOut; ++c) acc[c] = 0.0;
+
+  const int ntiles = (nj + kIeeeTile - 1) / kIeeeTile;
+  double2 buf[kPer];
+  auto fetch = [&](int t) {
+    const int nch = min(kIeeeTile, nj - t * kIeeeTile) * kRecChunks;
+    const double2 *src =
+        reinterpret_cast<const double2 *>(jr + (size_t)t * kIeeeTile);
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+      const int c = (int)threadIdx.x + p * 256;
+      if (c < nch) buf[p] = src[c];
+    }
+  };
+  auto store = [&](int b, int t) {
+    const int nch = min(kIeeeTile, nj - t * kIeeeTile) * kRecChunks;
+    double2 *dst = reinterpret_cast<double2 *>(&tile[b][0]);
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+      const int c = (int)threadIdx.x + p * 256;
+      if (c < nch) dst[c] = buf[p];
+    }
+  };
+
+  if (ntiles > 0) {
+    fetch(0);
+    store(0, 0);
+  }
+  __syncthreads();
+  for (int t = 0; t < ntiles; ++t) {
+    if (t + 1 < ntiles) fetch(t + 1);
+    if (active) {
+      const JRec *L = tile[t & 1];
+      const int nrec = min(kIeeeTile, nj - t * kIeeeTile);
+      int k = 0;
+      for (; k + U <= nrec; k += U) {
+        double tt[U][Pol::kOut];
+#pragma unroll
+        for (int u = 0; u < U; ++u) Pol::terms(q, L[k + u], tt[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) subtract_terms<Pol::kOut>(acc, tt[u]);
+      }
+      for (; k < nrec; ++k) {
+        double t1[Pol::kOut];
+        Pol::terms(q, L[k], t1);
+        subtract_terms<Pol::kOut>(acc, t1);
+      }
+    }
+    if (t + 1 < ntiles) store((t + 1) & 1, t + 1);
+    __syncthreads();
+  }
+  if (active && i0 + lane < ni) {
+#pragma unroll
+    for (int c = 0; c < Pol::kOut; ++c) out.f[c][i0 + lane] = acc[c];
+  }
+}
+
+template <class Pol, int U>
+int run_ieee_engine(size_t ni, const double *const *ip, size_t nj,
+                    const double *const *jp, double *const *op,
+                    hipStream_t st) {
+  if (ni == 0) return TUPAN_OK;
+  if (!ip || !jp || !op) return set_error(TUPAN_ERR_ARG, "NULL field array");
+  if (ni > (size_t)INT_MAX / 8 || nj > (size_t)INT_MAX / 8)
+    return set_error(TUPAN_ERR_ARG, "ni/nj too large (%zu, %zu)", ni, nj);
+  Soa is{}, js{};
+  for (int k = 0; k < Pol::kFields; ++k) {
+    if (!ip[k] || (nj && !jp[k]))
+      return set_error(TUPAN_ERR_ARG, "NULL input field %d", k);
+    is.f[k] = ip[k];
+    js.f[k] = jp[k];
+  }
+  Out out{};
+  for (int k = 0; k < Pol::kOut; ++k) {
+    if (!op[k]) return set_error(TUPAN_ERR_ARG, "NULL output field %d", k);
+    out.f[k] = op[k];
+  }
+  const int n_i = (int)ni, n_j = (int)nj;
+  typename Pol::JRec *jr = nullptr;
+  if (n_j > 0) {
+    const size_t bytes =
+        ((sizeof(typename Pol::JRec) * (size_t)n_j + 255) / 256) * 256;
+    const int rc = device_alloc((void **)&jr, bytes, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
+                       dim3(256), 0, st, js, n_j, jr, (const double *)nullptr);
+  }
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const bool timed = timing_begin(&ev0, &ev1);
+  if (timed) (void)hipEventRecord(ev0, st);
+  hipLaunchKernelGGL((ieee_engine<Pol, U>), dim3((n_i + 255) / 256),
+                     dim3(256), 0, st, is, n_i,
+                     (const typename Pol::JRec *)jr, n_j, out);
+  if (timed) (void)hipEventRecord(ev1, st);
+  hipError_t e = hipGetLastError();
+  device_free(jr, st);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return TUPAN_OK;
+}
 
 }  // namespace
 
@@ -208,17 +329,13 @@ int run_ieee(int kid, size_t ni, const double *const *ip, size_t nj,
              const double *const *jp, double *const *op, hipStream_t st) {
   switch (kid) {
     case TUPAN_K_PHI:
-      return run_engine<PhiIeee>(kid, TUPAN_MODE_FP64_IEEE, ni, ip, nj, jp,
-                                 op, {}, st);
+      return run_ieee_engine<PhiIeee, 4>(ni, ip, nj, jp, op, st);
     case TUPAN_K_ACC:
-      return run_engine<AccIeee>(kid, TUPAN_MODE_FP64_IEEE, ni, ip, nj, jp,
-                                 op, {}, st);
+      return run_ieee_engine<AccIeee, 4>(ni, ip, nj, jp, op, st);
     case TUPAN_K_ACC_JERK:
-      return run_engine<AccJerkIeee>(kid, TUPAN_MODE_FP64_IEEE, ni, ip, nj,
-                                     jp, op, {}, st);
+      return run_ieee_engine<AccJerkIeee, 4>(ni, ip, nj, jp, op, st);
     case TUPAN_K_SNAP_CRACKLE:
-      return run_engine<SnapCrackleIeee>(kid, TUPAN_MODE_FP64_IEEE, ni, ip,
-                                         nj, jp, op, {}, st);
+      return run_ieee_engine<SnapCrackleIeee, 2>(ni, ip, nj, jp, op, st);
   }
   return set_error(TUPAN_ERR_ARG, "no IEEE mode for kernel %d", kid);
 }

@@ -80,8 +80,6 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
   if (S > max_split && o.splits <= 0) S = max_split;
   if (S < 1) S = 1;
   if (nj > 0 && S > (long)nj) S = (long)nj;
-  // the reference's summation order: every j into one accumulator
-  if (mode == TUPAN_MODE_FP64_IEEE) S = 1;
   return LaunchShape{P, (int)S};
 }
 
