@@ -16,13 +16,18 @@
 //   * all 64 lanes walk the same j sequence, so a j record is wave-uniform:
 //     it is fetched with scalar loads (s_load) into SGPRs and used directly
 //     as a VALU operand -- no LDS round trip, no bank conflicts;
-//   * the j range is split S ways across blockIdx.y so that the chip holds
-//     >= 4 waves per SIMD at any N; the S partial sums land in an FP64
-//     scratch array and a finishing kernel adds them in fixed order
-//     (bit-reproducible run to run);
-//   * blocks are 256 threads (4 independent waves); consecutive blocks go
-//     round-robin to the 8 XCDs and all waves of one split read the same j
-//     records, which every XCD's L2 holds (<= 64 B * nj).
+//   * the j range is split into S = 4 * nrows chunks so that the chip
+//     holds >= 4 waves per SIMD at any N: a block's 4 waves take the same
+//     64*P i-particles and 4 consecutive chunks, and combine their sums
+//     through LDS in wave order; with nrows > 1 block rows the row partials
+//     land in an FP64 scratch array and a finishing kernel adds them in row
+//     order (fixed grouping: bit-reproducible run to run);
+//   * blockIdx.x runs over i-waves and blockIdx.y over block rows, so the
+//     blocks in flight read the same j chunks (scalar cache and L2 hits);
+//     blocks go round-robin to the 8 XCDs, so with a multiple of 8 i-waves
+//     each XCD keeps the same i-waves in every row.  (Mapping each XCD to a
+//     contiguous i range with rows innermost was measured: it breaks the
+//     j-chunk sharing and multiplied the HBM fetches 12x.)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -97,13 +102,19 @@ __device__ __forceinline__ uint32_t near_bit(uint64_t far) {
 template <class Pol, int P>
 __global__ __launch_bounds__(kBlock) void pair_engine(
     Soa is, Soa js, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
-    int jchunk, double *__restrict__ part, Out out,
+    int jchunk, int nrows, double *__restrict__ part, Out out,
     typename Pol::Params prm, const double *__restrict__ aux) {
   const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  const int i0 = tile * 64 * P;
-  if (i0 >= ni) return;  // wave-uniform; no block barriers below
-  const int s = blockIdx.y;
+  // wave-uniform by construction; readfirstlane lets the compiler keep the
+  // j range, and so the j-record loads, scalar
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // blockIdx.x: i-wave (fastest, so the blocks in flight share j chunks);
+  // blockIdx.y: block row of kWaves j chunks
+  const int row = blockIdx.y;
+  const int i0 = blockIdx.x * 64 * P;
+  if (i0 >= ni) return;  // block-uniform
+  // the block's waves take consecutive j chunks of the same i-particles
+  const int s = row * kWaves + wave;
 
   typename Pol::I ist[P];
   typename Pol::A64 acc[P];
@@ -178,25 +189,44 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
     }
   }
 
+  // Waves 1..3 hand their sums to wave 0 through LDS; wave 0 combines
+  // them in wave order (fixed grouping: bit-reproducible) and writes the
+  // block's result, or its partial when j is split over several blocks.
+  __shared__ double red[kWaves - 1][P][Pol::kOut][64];
+  if (wave > 0) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      double v[Pol::kOut];
+      Pol::values(acc[p], v);
+#pragma unroll
+      for (int k = 0; k < Pol::kOut; ++k) red[wave - 1][p][k][lane] = v[k];
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
 #pragma unroll
   for (int p = 0; p < P; ++p) {
+    double v[Pol::kOut];
+    Pol::values(acc[p], v);
+    for (int w = 0; w < kWaves - 1; ++w) {
+      double u[Pol::kOut];
+#pragma unroll
+      for (int k = 0; k < Pol::kOut; ++k) u[k] = red[w][p][k][lane];
+      Pol::combine(v, u);
+    }
     const int i = i0 + p * 64 + lane;
     if (i >= ni) continue;
-    if (gridDim.y == 1) {
-      double v[Pol::kOut];
-      Pol::values(acc[p], v);
+    if (nrows == 1) {
       Pol::finish(v, out, i, prm);
     } else {
-      double v[Pol::kOut];
-      Pol::values(acc[p], v);
 #pragma unroll
       for (int k = 0; k < Pol::kOut; ++k)
-        part[((size_t)s * Pol::kOut + k) * ni + i] = v[k];
+        part[((size_t)row * Pol::kOut + k) * ni + i] = v[k];
     }
   }
 }
 
-// Adds the S split partials in split order and applies the policy's
+// Adds the S block partials in block order and applies the policy's
 // finishing step (e.g. Tstep's eta / sqrt(1 + w2)).
 template <class Pol>
 __global__ __launch_bounds__(256) void finish_kernel(

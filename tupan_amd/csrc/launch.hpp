@@ -46,10 +46,10 @@ int sakura_budget();
 template <class Pol, int P>
 static void launch_p(dim3 grid, hipStream_t st, const Soa &is, const Soa &js,
                      int ni, const typename Pol::JRec *jr, int nj, int jchunk,
-                     double *part, const Out &out,
+                     int nrows, double *part, const Out &out,
                      const typename Pol::Params &prm, const double *aux) {
   hipLaunchKernelGGL((pair_engine<Pol, P>), grid, dim3(kBlock), 0, st, is,
-                     js, ni, jr, nj, jchunk, part, out, prm, aux);
+                     js, ni, jr, nj, jchunk, nrows, part, out, prm, aux);
 }
 
 template <class Pol, class = void>
@@ -86,14 +86,18 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   LaunchShape sh = choose_shape(kernel_id, mode, ni, nj, Pol::kPrefIpl,
                                 Pol::kMaxIpl);
   const int P = sh.ipl;
-  int S = sh.splits;
-  int jchunk = n_j;
-  if (S > 1) {
-    jchunk = (n_j + S - 1) / S;
-    jchunk = (jchunk + kTile - 1) / kTile * kTile;
-    S = (n_j + jchunk - 1) / jchunk;
-  }
-  if (S < 1) S = 1;
+  // S j chunks of jchunk records (a multiple of kTile), dealt kWaves per
+  // block: SB block rows (grid: i-waves x SB)
+  int S = sh.splits < kWaves ? kWaves : sh.splits;
+  S = (S + kWaves - 1) / kWaves * kWaves;
+  int jchunk = (n_j + S - 1) / S;
+  jchunk = (jchunk + kTile - 1) / kTile * kTile;
+  if (jchunk < kTile) jchunk = kTile;
+  const int used = (n_j + jchunk - 1) / jchunk;  // chunks holding records
+  const int SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
+  const int waves = (n_i + 64 * P - 1) / (64 * P);  // i-waves
+  if (SB > 65535)
+    return set_error(TUPAN_ERR_ARG, "too many j chunks (%d block rows)", SB);
 
   typename Pol::JRec *jr = nullptr;
   double *part = nullptr;
@@ -112,9 +116,9 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
       ctx = reinterpret_cast<double *>(reinterpret_cast<char *>(jr) +
                                        rec_bytes);
   }
-  if (S > 1) {
+  if (SB > 1) {
     rc = device_alloc((void **)&part,
-                      sizeof(double) * (size_t)S * Pol::kOut * (size_t)n_i,
+                      sizeof(double) * (size_t)SB * Pol::kOut * (size_t)n_i,
                       st);
     if (rc) {
       device_free(jr, st);
@@ -136,8 +140,7 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
                        dim3(256), 0, st, js, n_j, jr, (const double *)ctx);
   }
-  const int waves = (n_i + 64 * P - 1) / (64 * P);
-  dim3 grid((waves + kWaves - 1) / kWaves, S);
+  dim3 grid((unsigned)waves, (unsigned)SB);
   const double *aux = ctx ? ctx + 2 : nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   const bool timed = timing_begin(&ev0, &ev1);
@@ -145,19 +148,19 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   switch (P) {
     case 4:
       launch_p<Pol, (Pol::kMaxIpl >= 4 ? 4 : 1)>(grid, st, is, js, n_i, jr, n_j,
-                                                  jchunk, part, out, prm, aux);
+                                                  jchunk, SB, part, out, prm, aux);
       break;
     case 2:
       launch_p<Pol, (Pol::kMaxIpl >= 2 ? 2 : 1)>(grid, st, is, js, n_i, jr, n_j,
-                                                  jchunk, part, out, prm, aux);
+                                                  jchunk, SB, part, out, prm, aux);
       break;
     default:
-      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, part, out, prm, aux);
+      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, SB, part, out, prm, aux);
   }
   if (timed) (void)hipEventRecord(ev1, st);
-  if (S > 1)
+  if (SB > 1)
     hipLaunchKernelGGL((finish_kernel<Pol>), dim3((n_i + 255) / 256),
-                       dim3(256), 0, st, part, S, n_i, out, prm);
+                       dim3(256), 0, st, part, SB, n_i, out, prm);
   hipError_t e = hipGetLastError();
   device_free(part, st);
   device_free(jr, st);
