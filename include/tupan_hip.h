@@ -308,6 +308,28 @@ int tupan_pn_flux_dev(size_t n, const double *const *in, double *const *acc,
  * x = {rx..vz} += d = {drx..dvz}. */
 int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
                           void *stream);
+/* NREG's recentring (nreg.py:28-30, 54-56): dst[c] = src[c] / s for
+ * c < k (1 <= k <= 9), correctly rounded; dst may alias src. */
+int tupan_divide_dev(size_t n, int k, const double *const *src,
+                     double *const *dst, double s, void *stream);
+
+/* ---- reductions ---------------------------------------------------------
+ * No counterpart in the reference (it sums with numpy on the host).
+ * Deterministic (fixed tree for a given n: bit-reproducible run to run),
+ * written to a device array `out`.
+ *   tupan_sum_dev:         out[0] = sum_i x[i] (ps.x.sum(), e.g.
+ *                          nreg.py:26, 31, 57).
+ *   tupan_diagnostics_dev: the sums behind the report of
+ *                          simulation.py:96-129, one pass with body.py's
+ *                          per-particle expressions.  f: m, rx, ry, rz, vx,
+ *                          vy, vz, phi, and with pn != 0 also pn_ke,
+ *                          pn_mrx..z, pn_mvx..z, pn_amx..z.  out[12]: M,
+ *                          KE (with pn_ke), sum m phi (PE = half of it),
+ *                          sum m r (+pn_mr), sum m v (+pn_mv), sum r x m v
+ *                          (+pn_am). */
+int tupan_sum_dev(size_t n, const double *x, double *out, void *stream);
+int tupan_diagnostics_dev(size_t n, const double *const *f, int pn,
+                          double *out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,9 @@ numpy expression, evaluated the same way:
   SIA21.kdk/dkd, bridge_sf      sia.py:291-353
   SIA.recurse (shared steps)    sia.py:1087-1119
   sakura_step, get_sakura_tstep tupan/integrator/sakura.py:22-50, 100-117
+  nreg_x, nreg_v, anreg_step,   tupan/integrator/nreg.py:22-97, 114-134,
+  nreg_step, NREG.do_step       149-172
+  FewBody.drift/kepler/evolve   tupan/integrator/fewbody.py:17-57
 """
 import math
 import sys
@@ -74,6 +77,21 @@ class Kernels(object):
     def sakura(self, ps, dt, flag):
         _set(ps, ("drx", "dry", "drz", "dvx", "dvy", "dvz"),
              self.o.call("sakura_kernel", ps, ps, dt, flag))
+
+    def nreg_x(self, ps, dt):
+        _set(ps, ("mrx", "mry", "mrz", "ax", "ay", "az", "u"),
+             self.o.call("nreg_Xkernel", ps, ps, dt))
+
+    def nreg_v(self, ps, dt):
+        _set(ps, ("mvx", "mvy", "mvz", "mk"),
+             self.o.call("nreg_Vkernel", ps, ps, dt))
+
+    def kepler(self, ps, dt):
+        """kepler_solver_kernel in place on particles 0 and 1
+        (extensions.py:642-646)."""
+        new = self.o.kepler(ps, dt)
+        for name, v in zip(("rx", "ry", "rz", "vx", "vy", "vz"), new):
+            ps[name][:2] = v
 
 
 def copy(ps):
@@ -373,3 +391,90 @@ def energy(K, ps, pn=False):
         ke = ke + ps["pn_ke"]
     K.phi(ps)
     return float(ke.sum()) + 0.5 * float((ps["mass"] * ps["phi"]).sum())
+
+
+# ---- NREG (nreg.py) -----------------------------------------------------
+# The scalars the reference keeps as class attributes of the particle type
+# (E0, W, U, S) live in the dict ``st``.
+
+def nreg_initialize(K, ps, st):
+    """NREG.initialize (nreg.py:114-126): E0 = KE + PE, W = U = S = -PE."""
+    ke = float((0.5 * ps["mass"] * (ps["vx"]**2 + ps["vy"]**2
+                                    + ps["vz"]**2)).sum())
+    K.phi(ps)
+    pe = 0.5 * float((ps["mass"] * ps["phi"]).sum())
+    st["E0"] = ke + pe
+    st["W"] = -pe
+    st["U"] = -pe
+    st["S"] = -pe
+
+
+def nreg_x(K, ps, st, dt):
+    """nreg.py:22-33: positions at the drifted separations, recentred."""
+    mtot = float(ps["mass"].sum())
+    K.nreg_x(ps, dt)
+    ps["rx"] = ps["mrx"] / mtot
+    ps["ry"] = ps["mry"] / mtot
+    ps["rz"] = ps["mrz"] / mtot
+    st["U"] = 0.5 * ps["u"].sum()
+    ps["t_curr"] += dt
+
+
+def nreg_v(K, ps, st, dt):
+    """nreg.py:45-59: velocities after the kick, recentred; W = K - E0."""
+    mtot = float(ps["mass"].sum())
+    K.nreg_v(ps, dt)
+    ps["vx"] = ps["mvx"] / mtot
+    ps["vy"] = ps["mvy"] / mtot
+    ps["vz"] = ps["mvz"] / mtot
+    kin = 0.25 * ps["mk"].sum() / mtot
+    st["W"] = kin - st["E0"]
+
+
+def anreg_step(K, ps, st, h):
+    """nreg.py:77-85: drift(h/2W) kick(h/U) drift(h/2W), each step length
+    taken from the scalars as they stand when it starts."""
+    nreg_x(K, ps, st, 0.5 * (h / st["W"]))
+    nreg_v(K, ps, st, (h / st["U"]))
+    nreg_x(K, ps, st, 0.5 * (h / st["W"]))
+
+
+def nreg_step(K, ps, st, h):
+    """nreg.py:88-97."""
+    anreg_step(K, ps, st, 0.5 * (h * st["S"]))
+    st["S"] = 1 / (2 / st["W"] - 1 / st["S"])
+    anreg_step(K, ps, st, 0.5 * (h * st["S"]))
+
+
+def nreg_do_step(K, ps, st, tau, method):
+    """NREG.do_step (nreg.py:149-163): tstep = the physical time advanced,
+    time += tau (the regularised step), nstep += 1."""
+    t0 = ps["t_curr"]
+    if "anreg" in method:
+        anreg_step(K, ps, st, tau / 2)
+    else:
+        nreg_step(K, ps, st, tau)
+    dt = ps["t_curr"] - t0
+    n = len(ps["mass"])
+    ps["tstep"] = np.full(n, dt)
+    ps["time"] = ps.get("time", np.zeros(n)) + tau
+    ps["nstep"] = ps.get("nstep", np.zeros(n, np.uint64)) + 1
+    return dt
+
+
+# ---- FewBody (fewbody.py) ------------------------------------------------
+
+def fewbody_evolve(K, ps, tau, pn=False):
+    """FewBody.evolve: nothing for n = 0, a drift for n = 1, the Kepler
+    solver on particles 0 and 1 otherwise (fewbody.py:17-57)."""
+    n = len(ps["mass"])
+    if n == 0:
+        return
+    if n == 1:
+        drift(K, ps, tau, pn)
+        return
+    if pn:
+        raise NotImplementedError("The current version of the "
+                                  "Kepler-solver does not include "
+                                  "post-Newtonian corrections.")
+    K.kepler(ps, tau)

@@ -1,0 +1,178 @@
+"""Device sums, the fused report, NREG and FewBody against the checker.
+
+* tupan_sum_dev against the exact sum (math.fsum): within 2 log2(n) FP64
+  ulps of sum |x| (a summation tree's error bound), at sizes straddling the
+  block and grid boundaries;
+* System.diagnostics against the checker's restatement of body.py: totals
+  within 32 ulps of the sum of the absolute terms, the potential energy
+  within 1e-12 (it goes through the FP64 Phi kernel), with and without the
+  PN accumulators; the Diagnostic report's eerr/geerr/com/momentum columns;
+* NREG (nreg, anreg) on a 64-body Plummer model, 6 steps, against the
+  checker's NREG on the reference's C kernels: positions, velocities and
+  the physical clock within 1e-11;
+* FewBody: n = 1 drift and n = 2/3 Kepler solve against the checker.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import Oracle  # noqa: E402
+from oracle import diagnostics as D  # noqa: E402
+from oracle import integrators as I  # noqa: E402
+from tupan_amd import _lib, kernels  # noqa: E402
+
+EPS = 2.0**-52
+BASE = ("mass", "eps2", "rx", "ry", "rz", "vx", "vy", "vz")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device_present():
+    assert _lib.device_count() >= 1, _lib.lib().tupan_last_error()
+
+
+@pytest.fixture(scope="module")
+def K():
+    return I.Kernels(Oracle("float64", "port"))
+
+
+def _plummer(n, seed, soft):
+    from tupan_amd.ics import make_plummer
+    phi = lambda ps: kernels.compute("phi", ps, ps, mode="fp64")[0]
+    ps = make_plummer(n, soft, seed=seed, phi_fn=phi)
+    return {f: np.ascontiguousarray(getattr(ps, f), np.float64) for f in BASE}
+
+
+def _system(d):
+    from tupan_amd.integrator import System
+    return System.from_host(d)
+
+
+@pytest.mark.parametrize("n", [0, 1, 255, 256, 257, 65539, 1000003])
+def test_sum_matches_exact_sum(n):
+    import torch
+    rng = np.random.RandomState(n % 97)
+    x = rng.standard_normal(n) * np.exp(rng.uniform(-5, 5, n))
+    t = torch.as_tensor(x, device="cuda")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call_update("sum", n, t.data_ptr(), out.data_ptr(), 0)
+    got = float(out.item())
+    exact = math.fsum(x)
+    bound = 2 * max(1, math.log2(max(n, 2))) * EPS * float(np.abs(x).sum())
+    assert abs(got - exact) <= bound, (n, got, exact)
+    again = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call_update("sum", n, t.data_ptr(), again.data_ptr(), 0)
+    assert float(again.item()) == got  # fixed tree: bit-reproducible
+
+
+@pytest.mark.parametrize("pn", [False, True])
+def test_diagnostics_match_checker(K, pn):
+    import torch
+    d = _plummer(3001, 11, 0.01)
+    rng = np.random.RandomState(4)
+    s = _system(d)
+    host = dict(d)
+    if pn:
+        s.include_pn_corrections = True
+        for f in D.PN_FIELDS:
+            v = 1e-4 * rng.standard_normal(3001)
+            s.field(f).copy_(torch.as_tensor(v))
+            host[f] = v
+    got = s.diagnostics()
+    ref = D.quantities(K, host, pn)
+    m = host["mass"]
+    scale = {"total_mass": m.sum(),
+             "kinetic_energy": (0.5 * m * (host["vx"]**2 + host["vy"]**2
+                                           + host["vz"]**2)).sum()
+             + (np.abs(host["pn_ke"]).sum() if pn else 0.0)}
+    for k in ("total_mass", "kinetic_energy"):
+        assert abs(got[k] - ref[k]) <= 32 * EPS * scale[k], k
+    assert abs(got["potential_energy"] - ref["potential_energy"]) \
+        <= 1e-12 * abs(ref["potential_energy"])
+    absr = sum(np.abs(m * host["r" + c]).sum() for c in "xyz")
+    absv = sum(np.abs(m * host["v" + c]).sum() for c in "xyz")
+    absl = sum(np.abs(host[a] * m * host[b]).sum()
+               for a in ("rx", "ry", "rz") for b in ("vx", "vy", "vz"))
+    if pn:
+        absr += sum(np.abs(host["pn_mr" + c]).sum() for c in "xyz")
+        absv += sum(np.abs(host["pn_mv" + c]).sum() for c in "xyz")
+        absl += sum(np.abs(host["pn_am" + c]).sum() for c in "xyz")
+    M = ref["total_mass"]
+    assert np.abs(got["com_r"] - ref["com_r"]).max() <= 64 * EPS * absr / M
+    assert np.abs(got["com_v"] - ref["com_v"]).max() <= 64 * EPS * absv / M
+    assert np.abs(got["linear_momentum"]
+                  - ref["linear_momentum"]).max() <= 64 * EPS * absv
+    assert np.abs(got["angular_momentum"]
+                  - ref["angular_momentum"]).max() <= 64 * EPS * absl
+
+
+def test_report_columns_match_checker(K):
+    from tupan_amd.diagnostics import Diagnostic
+    from tupan_amd.integrator import Integrator
+    d = _plummer(256, 12, 0.01)
+    s = _system(d)
+    integ = Integrator(1.0 / 64, 0.0, s, "hermite4")
+    rep = Diagnostic(None, 0.0, s, report_freq=1)
+    rep.initialize(s)
+    host = dict(d)
+    host["t_curr"] = 0.0
+    ref = D.Diagnostic(K, host)
+    rows = []
+    for _ in range(4):
+        integ.evolve_step(1.0)
+        I.hermite_step(K, host, 1.0 / 64, 4)
+        rows.append((rep.diagnostic_report(s), ref.report(host)))
+    assert rows[0][0]["eerr"] != 0.0
+    for got, want in rows:
+        for k in ("ke", "pe", "te", "virial"):
+            assert abs(got[k] - want[k]) <= 1e-12 * abs(want[k]), k
+        assert abs(got["eerr"] - want["eerr"]) <= 1e-11
+        for k in ("com_r", "com_v", "lmom", "amom"):
+            assert abs(got[k] - want[k]) <= 1e-12, k
+
+
+@pytest.mark.parametrize("method", ["nreg", "anreg"])
+def test_nreg_matches_checker(K, method):
+    from tupan_amd.integrator import Integrator
+    d = _plummer(64, 13, 1e-3)
+    s = _system(d)
+    integ = Integrator(1.0 / 32, 0.0, s, method)
+    host = dict(d)
+    host["t_curr"] = 0.0
+    st = {}
+    I.nreg_initialize(K, host, st)
+    for k in range(6):
+        integ.evolve_step(10.0)
+        I.nreg_do_step(K, host, st, 1.0 / 32, method)
+        got = s.to_host(BASE)
+        scale = max(np.abs(host[f]).max() for f in ("rx", "ry", "rz"))
+        for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+            assert np.abs(got[f] - host[f]).max() <= 1e-11 * scale, (k, f)
+        assert abs(s.t_curr - host["t_curr"]) \
+            <= 1e-11 * (abs(host["t_curr"]) + 1e-3)
+    tstep = s.tstep.cpu().numpy()
+    assert np.all(tstep == tstep[0])
+    assert abs(tstep[0] - host["tstep"][0]) <= 1e-11 * abs(host["tstep"][0])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+def test_fewbody_matches_checker(K, n):
+    import torch
+    from tupan_amd.integrator import FewBody
+    rng = np.random.RandomState(20 + n)
+    d = {f: rng.standard_normal(n) for f in BASE}
+    d["mass"] = rng.uniform(0.5, 1.5, n)
+    d["eps2"] = np.full(n, 1e-4)
+    s = _system(d)
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    FewBody.evolve(s, 0.37, status)
+    torch.cuda.synchronize()
+    assert int(status.item()) == 0
+    host = dict(d)
+    I.fewbody_evolve(K, host, 0.37)
+    got = s.to_host(BASE)
+    for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        tol = 0.0 if n == 1 else 1e-11 * (np.abs(host[f]).max() + 1.0)
+        assert np.abs(got[f] - host[f]).max() <= tol, (n, f)

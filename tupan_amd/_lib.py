@@ -106,6 +106,10 @@ def lib():
         L.tupan_kick_pn_half_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
         L.tupan_pn_flux_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
         L.tupan_apply_delta_dev.argtypes = [_SZ, _PP, _PP, _P]
+        L.tupan_divide_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP,
+                                       ctypes.c_double, _P]
+        L.tupan_sum_dev.argtypes = [_SZ, _P, _P, _P]
+        L.tupan_diagnostics_dev.argtypes = [_SZ, _PP, ctypes.c_int, _P, _P]
         for name in ("nreg_x", "nreg_v"):
             getattr(L, "tupan_%s_dev" % name).argtypes = [
                 _SZ, _PP, _SZ, _PP, _D, _PP, _P]

@@ -24,7 +24,7 @@ ROOT = os.path.dirname(HERE)
 
 HIP_SOURCES = ["runtime.hip", "gravity.hip", "pnacc.hip", "sakura.hip",
                "integrate.hip", "adjoint.hip", "ensemble.hip", "nreg.hip",
-               "ieee.hip"]
+               "ieee.hip", "reduce.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TUPAN_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",

@@ -16,6 +16,7 @@
 //                    (pn_kick_ke, pn_kick_lmom, pn_kick_amom) and
 //                    body.py:481-492 (pn_drift_com_r)
 //   apply_delta      sakura.py:31-36, 39-44
+//   divide           nreg.py:28-30, 54-56 (r = mr / M, v = mv / M)
 #include "hermite.hpp"
 #include "launch.hpp"
 
@@ -167,6 +168,16 @@ __global__ void __launch_bounds__(kThreads)
        i += gridDim.x * kThreads) {
 #pragma unroll
     for (int k = 0; k < 6; ++k) p.f[k][i] = p.f[k][i] + p.f[6 + k][i];
+  }
+}
+
+// dst[k][i] = src[k][i] / s for k < nk (correctly rounded division)
+__global__ void __launch_bounds__(kThreads)
+    divide_kernel(int n, int nk, P18 p, double s) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+    for (int k = 0; k < nk; ++k) p.f[9 + k][i] = p.f[k][i] / s;
   }
 }
 
@@ -358,6 +369,25 @@ int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
   hipLaunchKernelGGL(apply_delta_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
                      reinterpret_cast<hipStream_t>(stream), (int)n, p);
   return finish("apply_delta");
+}
+
+int tupan_divide_dev(size_t n, int k, const double *const *src,
+                     double *const *dst, double s, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if (k < 1 || k > 9) return set_error(TUPAN_ERR_ARG, "k must be in 1..9");
+  if ((rc = check_fields((const void *const *)src, k, "source"))) return rc;
+  if ((rc = check_fields((const void *const *)dst, k, "destination")))
+    return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{};
+  for (int c = 0; c < k; ++c) {
+    p.f[c] = const_cast<double *>(src[c]);
+    p.f[9 + c] = dst[c];
+  }
+  hipLaunchKernelGGL(divide_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, k, p, s);
+  return finish("divide");
 }
 
 }  // extern "C"

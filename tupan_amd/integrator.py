@@ -16,12 +16,14 @@ Provided (the reference's method names):
   Sakura   sakura, asakura                    (tupan/integrator/sakura.py)
   SakuraPN sakura_pn: close pairs by the Sakura Kepler drift, the rest and
            the PN terms by kicks (a hybrid the reference does not have)
+  NREG     nreg, anreg                        (tupan/integrator/nreg.py)
+  FewBody  drift / two-body Kepler solve      (tupan/integrator/fewbody.py)
 
 Differences from the reference, all deliberate:
   * the state is not copied per step (``ps.copy()``, hermite.py:79): the
     predictor writes the snapshot into a preallocated shadow set;
-  * hierarchical SIA (``sia21h``) and the few-body solver for n <= 2
-    (sia.py:32, fewbody.py) are not provided: they raise;
+  * hierarchical SIA (``sia21h``, which hands subsystems of n <= 2 to
+    FewBody, sia.py:32) is not provided: it raises;
   * a Sakura pair that exhausts its Kepler-solve budget raises
     ``NotConverged`` at the end of ``evolve`` (the reference loops).
 """
@@ -127,20 +129,57 @@ class System(object):
     def set_sakura(self, dt, flag, status=None):
         return self._pair("sakura", dt, flag, status=status)
 
-    # ---- diagnostics (body.py:259-321) ----------------------------------
+    # ---- sums and diagnostics (body.py:79-322) ---------------------------
+    def sum(self, name):
+        """Deterministic device sum of one field (``ps.<name>.sum()``)."""
+        out = torch.empty(1, dtype=torch.float64, device=self.device)
+        _lib.call_update("sum", self.n, self.field(name).data_ptr(),
+                         out.data_ptr(), _stream(self))
+        return float(out.item())
+
+    def total_mass(self):
+        return self.sum("mass")
+
+    def diagnostics(self, with_phi=True):
+        """The totals of the reference's report (body.py:79-322: total
+        mass, kinetic, potential and virial energy, centre-of-mass position
+        and velocity, linear and angular momentum, PN accumulators included
+        when enabled) from one fused device pass; ``with_phi`` first
+        evaluates Phi in FP64 (``ps.set_phi(ps)``, body.py:298)."""
+        if with_phi:
+            self.set_phi(mode="fp64")
+        names = ("mass", "rx", "ry", "rz", "vx", "vy", "vz", "phi")
+        pn = bool(self.include_pn_corrections)
+        if pn:
+            names += ("pn_ke", "pn_mrx", "pn_mry", "pn_mrz", "pn_mvx",
+                      "pn_mvy", "pn_mvz", "pn_amx", "pn_amy", "pn_amz")
+        out = torch.empty(12, dtype=torch.float64, device=self.device)
+        _lib.call_update("diagnostics", self.n, self.ptrs(names), int(pn),
+                         out.data_ptr(), _stream(self))
+        s = out.cpu().numpy()
+        mtot = float(s[0])
+        ke = float(s[1])
+        pe = 0.5 * float(s[2])
+        return {
+            "total_mass": mtot,
+            "kinetic_energy": ke,
+            "potential_energy": pe,
+            "virial_energy": 2 * ke + pe,
+            "com_r": s[3:6] / mtot,
+            "com_v": s[6:9] / mtot,
+            "linear_momentum": s[6:9].copy(),
+            "angular_momentum": s[9:12].copy(),
+        }
+
     def kinetic_energy(self):
-        ke = 0.5 * self.mass * (self.vx * self.vx + self.vy * self.vy
-                                + self.vz * self.vz)
-        if self.include_pn_corrections:
-            ke = ke + self.field("pn_ke")
-        return float(ke.sum())
+        return self.diagnostics(with_phi=False)["kinetic_energy"]
 
     def potential_energy(self):
-        self.set_phi(mode="fp64")
-        return 0.5 * float((self.mass * self.phi).sum())
+        return self.diagnostics()["potential_energy"]
 
     def energy(self):
-        return self.kinetic_energy() + self.potential_energy()
+        d = self.diagnostics()
+        return d["kinetic_energy"] + d["potential_energy"]
 
     def min_tstep(self):
         return float(self.tstep.abs().min())
@@ -452,14 +491,129 @@ class SakuraPN(SIA):
         return ps
 
 
+class NREG(Base):
+    """Algorithmic-regularisation leapfrog (tupan/integrator/nreg.py:22-172):
+    drift(h/2W) kick(h/U) drift(h/2W) with the pair sums of the NREG kernels
+    at the drifted separations and kicked velocities, positions and
+    velocities recentred on the centre of mass after each sub-step.  The
+    scalars the reference keeps on the particle type (E0, W, U, S) live on
+    the integrator; each sub-step's length needs the last one's sums, which
+    come back to the host as in the reference."""
+
+    PROVIDED_METHODS = ["nreg", "anreg"]
+
+    def __init__(self, eta, time, ps, method, **kw):
+        super(NREG, self).__init__(eta, time, ps, method, **kw)
+        if ps.include_pn_corrections:
+            raise NotImplementedError("NREG has no post-Newtonian terms")
+        self.is_initialized = False
+
+    def initialize(self):
+        """nreg.py:114-126: E0 = KE + PE, W = U = S = -PE."""
+        d = self.ps.diagnostics()
+        self.E0 = d["kinetic_energy"] + d["potential_energy"]
+        self.W = self.U = self.S = -d["potential_energy"]
+        self.is_initialized = True
+
+    def evolve_step(self, t_end):
+        if not self.is_initialized:
+            self.initialize()
+        super(NREG, self).evolve_step(t_end)
+
+    def nreg_x(self, ps, dt):
+        """nreg.py:22-33."""
+        mtot = ps.total_mass()
+        ps._pair("nreg_x", dt)
+        _lib.call_update("divide", ps.n, 3, ps.ptrs(("mrx", "mry", "mrz")),
+                         ps.ptrs(("rx", "ry", "rz")), float(mtot), _stream(ps))
+        self.U = 0.5 * ps.sum("u")
+        ps.t_curr += dt
+
+    def nreg_v(self, ps, dt):
+        """nreg.py:45-59."""
+        mtot = ps.total_mass()
+        ps._pair("nreg_v", dt)
+        _lib.call_update("divide", ps.n, 3, ps.ptrs(("mvx", "mvy", "mvz")),
+                         ps.ptrs(("vx", "vy", "vz")), float(mtot), _stream(ps))
+        kin = 0.25 * ps.sum("mk") / mtot
+        self.W = kin - self.E0
+
+    def anreg_step(self, ps, h):
+        """nreg.py:77-85."""
+        self.nreg_x(ps, 0.5 * (h / self.W))
+        self.nreg_v(ps, (h / self.U))
+        self.nreg_x(ps, 0.5 * (h / self.W))
+
+    def nreg_step(self, ps, h):
+        """nreg.py:88-97."""
+        self.anreg_step(ps, 0.5 * (h * self.S))
+        self.S = 1 / (2 / self.W - 1 / self.S)
+        self.anreg_step(ps, 0.5 * (h * self.S))
+
+    def do_step(self, ps, tau):
+        """nreg.py:149-163."""
+        t0 = ps.t_curr
+        if "anreg" in self.method:
+            self.anreg_step(ps, tau / 2)
+        else:
+            self.nreg_step(ps, tau)
+        ps.tstep.fill_(ps.t_curr - t0)
+        ps.time.add_(tau)
+        ps.nstep.add_(1)
+        return ps
+
+
+class FewBody(object):
+    """tupan/integrator/fewbody.py:14-57 on a device System: nothing for
+    n = 0, a drift for n = 1, the two-body Kepler solver (particles 0 and 1,
+    in place) otherwise."""
+
+    @staticmethod
+    def drift(ps, tau):
+        """fewbody.py:17-28 (with pn_drift_com_r when PN is on)."""
+        pn = ps.include_pn_corrections
+        _lib.call_update(
+            "drift", ps.n, ps.ptrs(("rx", "ry", "rz")),
+            ps.ptrs(("vx", "vy", "vz")), float(tau), 0,
+            ps.ptrs(("pn_mrx", "pn_mry", "pn_mrz")) if pn else None,
+            ps.ptrs(("pn_mvx", "pn_mvy", "pn_mvz")) if pn else None,
+            _stream(ps))
+        return ps
+
+    @staticmethod
+    def kepler_solver(ps, tau, status=None):
+        """fewbody.py:30-43; ``status`` (device int32) is OR-ed with 1 when
+        the solve hit its iteration budget."""
+        if ps.include_pn_corrections:
+            raise NotImplementedError("The current version of the "
+                                      "Kepler-solver does not include "
+                                      "post-Newtonian corrections.")
+        fields = ps.ptrs(spec.KEPLER_FIELDS)
+        outs = ps.ptrs(spec.KEPLER_OUTS)
+        _lib.call_update("kepler", fields, float(tau), outs,
+                         status.data_ptr() if status is not None else None,
+                         _stream(ps))
+        return ps
+
+    @staticmethod
+    def evolve(ps, tau, status=None):
+        """fewbody.py:45-57."""
+        if ps.n == 0:
+            return ps
+        if ps.n == 1:
+            return FewBody.drift(ps, tau)
+        return FewBody.kepler_solver(ps, tau, status)
+
+
 class Integrator(object):
     """Method-name dispatch (tupan/integrator/__init__.py:101-160)."""
 
     PROVIDED_METHODS = (Hermite.PROVIDED_METHODS + SIA.PROVIDED_METHODS
-                        + Sakura.PROVIDED_METHODS + SakuraPN.PROVIDED_METHODS)
+                        + Sakura.PROVIDED_METHODS + SakuraPN.PROVIDED_METHODS
+                        + NREG.PROVIDED_METHODS)
 
     def __init__(self, eta, time, ps, method=None, **kw):
-        for cls in (Hermite, SIA, Sakura, SakuraPN):
+        for cls in (Hermite, SIA, Sakura, SakuraPN, NREG):
             if method in cls.PROVIDED_METHODS:
                 self.integrator = cls(eta, time, ps, method, **kw)
                 break
