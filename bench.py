@@ -55,14 +55,16 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--workload", default="accjerk",
-                   choices=["accjerk", "sizes", "ensemble", "pn"],
+                   choices=["accjerk", "sizes", "ensemble", "pn", "diag",
+                            "nreg"],
                    help="accjerk: the headline single-evaluation metric; "
                         "sizes: Acc, AccJerk, SnapCrackle at N = 16K, 64K, "
                         "256K in both modes with errors vs FP64; "
                         "ensemble: the 512-cluster ragged ensemble (adaptive "
                         "Hermite-4) against looping the single-system path; "
                         "pn: the compact post-Newtonian cluster with Sakura "
-                        "close pairs")
+                        "close pairs; diag: the energy/momentum report at "
+                        "N = 65536; nreg: NREG steps at N = 1024")
     p.add_argument("--pmc-target", action="store_true",
                    help="one evaluation per mode plus a 256 MiB device copy "
                         "(the byte-counter calibration kernel), for "
@@ -323,14 +325,9 @@ def bench_pn(args, torch):
                           clight=c)
 
     def invariants(s):
-        m = s.mass
-        r = torch.stack([s.rx, s.ry, s.rz])
-        v = torch.stack([s.vx, s.vy, s.vz])
-        P = (m * v).sum(1) + torch.stack([s.field("pn_mv" + d).sum()
-                                          for d in "xyz"])
-        L = (m * torch.linalg.cross(r, v, dim=0)).sum(1) + torch.stack(
-            [s.field("pn_am" + d).sum() for d in "xyz"])
-        return s.energy(), P.cpu().numpy(), L.cpu().numpy()
+        d = s.diagnostics()   # PN accumulators included
+        return (d["kinetic_energy"] + d["potential_energy"],
+                d["linear_momentum"], d["angular_momentum"])
 
     E0, P0, L0 = invariants(ps)
     x = np.stack([p.rx, p.ry, p.rz])
@@ -390,6 +387,128 @@ def bench_pn(args, torch):
     print(json.dumps(result))
 
 
+def bench_diag(args, torch):
+    """The energy/momentum report (tupan/simulation.py:96-129) on the N =
+    65536 benchmark model: one FP64 Phi evaluation plus the fused device
+    reduction (System.diagnostics), against the reference's path on one
+    host core -- its C Phi kernel (timed on a sample of i-rows, scaled to N)
+    plus the report's numpy reductions (timed in full)."""
+    from tupan_amd import integrator as ig
+    n = 65536
+    p = make_workload(n)
+    ps = ig.System.from_host(p)
+    for _ in range(2):
+        ps.diagnostics()
+    torch.cuda.synchronize()
+    reps = max(3, args.steps // 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d = ps.diagnostics()
+    elapsed = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ps.diagnostics(with_phi=False)
+    red = (time.perf_counter() - t0) / reps
+    import oracle
+    from oracle import diagnostics as OD
+    kind = "reference" if oracle.available("float64", "reference") else "port"
+    o = oracle.Oracle("float64", kind)
+    h = {f: np.ascontiguousarray(getattr(p, f)) for f in
+         ("mass", "eps2", "rx", "ry", "rz", "vx", "vy", "vz")}
+    rows = 2048
+    idx = np.sort(np.random.RandomState(0).choice(n, rows, replace=False))
+    t = time.perf_counter()
+    o.call("phi_kernel", {f: v[idx] for f, v in h.items()}, h)
+    phi_cpu = (time.perf_counter() - t) * n / rows
+
+    phi_host = ps.phi.cpu().numpy()
+
+    class _K(object):  # Phi is timed above: here the reductions only
+        def phi(self, q):
+            q["phi"] = phi_host
+    t = time.perf_counter()
+    q = OD.quantities(_K(), dict(h))
+    numpy_cpu = time.perf_counter() - t
+    cpu = phi_cpu + numpy_cpu
+    result = {
+        "metric": "energy/momentum report time, N=65536 (Phi + reductions)",
+        "value": 1e3 * elapsed, "unit": "ms", "n_gpus": 1, "steps": reps,
+        "warmup": 2, "ms_per_step": 1e3 * elapsed, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": "synthetic",
+        "config": {"workload": "diagnostic report on the Plummer N=65536 "
+                               "seed 1 Salpeter model (FP64 Phi + fused "
+                               "reduction of 12 totals)"},
+        "reduction_ms": 1e3 * red,
+        "energy": d["kinetic_energy"] + d["potential_energy"],
+        "energy_host": q["kinetic_energy"] + q["potential_energy"],
+        "cpu_baseline": {"value": 1e3 * cpu, "unit": "ms", "cores": 1,
+                         "kind": kind,
+                         "sample": "C phi_kernel on %d sampled i-rows x all "
+                                   "%d j (scaled: %.1f s) + numpy report "
+                                   "reductions in full (%.2f ms), 1 thread"
+                                   % (rows, n, phi_cpu, 1e3 * numpy_cpu)},
+    }
+    print(json.dumps(result))
+
+
+def bench_nreg(args, torch):
+    """NREG (anreg, the algorithmic-regularisation leapfrog of
+    tupan/integrator/nreg.py) on a Plummer N=1024 model (seed 1, equal
+    masses, eps = 4/N): regularised steps per second on the device against
+    the reference's C NREG kernels driven by the reference's numpy step
+    (the checker's restatement) on one host core."""
+    from tupan_amd import integrator as ig
+    from tupan_amd import kernels
+    from tupan_amd.ics import make_plummer
+    n, h, steps = 1024, 1.0 / 64, max(16, args.steps)
+    phi = lambda q: kernels.compute("phi", q, q, mode="fp64")[0]
+    p = make_plummer(n, 4.0 / n, seed=1, phi_fn=phi)
+    ps = ig.System.from_host(p)
+    integ = ig.Integrator(h, 0.0, ps, "anreg")
+    integ.evolve_step(1e9)
+    torch.cuda.synchronize()
+    e0 = integ.integrator.E0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        integ.evolve_step(1e9)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    d = ps.diagnostics()
+    e1 = d["kinetic_energy"] + d["potential_energy"]
+    import oracle
+    from oracle import integrators as OI
+    kind = "reference" if oracle.available("float64", "reference") else "port"
+    K = OI.Kernels(oracle.Oracle("float64", kind))
+    host = {f: np.ascontiguousarray(getattr(p, f)) for f in
+            ("mass", "eps2", "rx", "ry", "rz", "vx", "vy", "vz")}
+    host["t_curr"] = 0.0
+    st = {}
+    OI.nreg_initialize(K, host, st)
+    t = time.perf_counter()
+    cpu_steps = 0
+    while time.perf_counter() - t < args.cpu_seconds / 2 or cpu_steps == 0:
+        OI.nreg_do_step(K, host, st, h, "anreg")
+        cpu_steps += 1
+    cpu_rate = cpu_steps / (time.perf_counter() - t)
+    result = {
+        "metric": "NREG (anreg) steps/s, Plummer N=1024",
+        "value": steps / elapsed, "unit": "steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": 1, "ms_per_step": 1e3 * elapsed / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": "anreg, Plummer N=1024 seed 1, equal masses, "
+                               "eps=4/N, h=1/64 (2 NREG X + 1 NREG V pair "
+                               "sums per step)"},
+        "energy_rel_err": abs(e1 - e0) / abs(e0),
+        "cpu_baseline": {"value": cpu_rate, "unit": "steps/s", "cores": 1,
+                         "kind": kind,
+                         "sample": "%d anreg steps, C NREG kernels + numpy "
+                                   "step, 1 thread" % cpu_steps},
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -410,6 +529,12 @@ def main():
         return
     if args.workload == "sizes":
         bench_sizes(args, torch)
+        return
+    if args.workload == "diag":
+        bench_diag(args, torch)
+        return
+    if args.workload == "nreg":
+        bench_nreg(args, torch)
         return
     n = args.n
     print("[bench] building Plummer N=%d workload" % n, file=sys.stderr, flush=True)
