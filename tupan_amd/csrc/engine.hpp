@@ -98,7 +98,7 @@ __device__ __forceinline__ uint32_t near_bit(uint64_t far) {
 
 // ------------------------------------------------------------- engine
 // FP32C policies (kTwoLevel) also receive the caller's FP64 arrays and the
-// j index: a pair they judge "near" is re-evaluated from those in FP64.
+// j index: a pair they classify as "near" is re-evaluated from those in FP64.
 template <class Pol, int P>
 __global__ __launch_bounds__(kBlock) void pair_engine(
     Soa is, Soa js, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
