@@ -33,8 +33,9 @@
  *                               contraction, correctly rounded division and
  *                               sqrt, j summed in ascending order (one j
  *                               split); results equal the reference's bits
- *       (modes apply to Phi, Acc, AccJerk, SnapCrackle; Tstep, PNAcc, Sakura
- *       and NREG always run FP64 and take no mode).
+ *       (modes apply to Phi, Acc, AccJerk, SnapCrackle and Tstep -- which
+ *       has no FP32C variant and runs FP64 there; PNAcc, Sakura and NREG
+ *       always run FP64 and take no mode).
  *
  * Every function returns 0 on success or a TUPAN_ERR_* code; the text of the
  * last error on the calling thread is tupan_last_error().
@@ -133,11 +134,12 @@ int tupan_snap_crackle_host(int mode, size_t ni, const double *const *ip,
                             double *const *op);
 
 /* ---- Tstep: replaces tstep_kernel, libtupan.h:98-119 -----------------
- * ip, jp: F8.  op: dt_a (-> tstep), dt_b (-> tstepij). */
-int tupan_tstep_dev(size_t ni, const double *const *ip, size_t nj,
+ * ip, jp: F8.  op: dt_a (-> tstep), dt_b (-> tstepij).  mode: FP64 (FP32C
+ * has no Tstep variant and runs FP64) or FP64_IEEE. */
+int tupan_tstep_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                     const double *const *jp, double eta, double *const *op,
                     void *stream);
-int tupan_tstep_host(size_t ni, const double *const *ip, size_t nj,
+int tupan_tstep_host(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double eta, double *const *op);
 
 /* ---- PNAcc: replaces pnacc_kernel, libtupan.h:121-150 ----------------

@@ -1,6 +1,6 @@
 """The reference-arithmetic mode (fp64_ieee) on the GPU: bit for bit.
 
-Phi, Acc, AccJerk and SnapCrackle in fp64_ieee perform the reference C's
+Phi, Acc, AccJerk, SnapCrackle and Tstep in fp64_ieee perform the reference C's
 operations in its order with its rounding (no FMA contraction, correctly
 rounded division and sqrt, j summed in ascending order), so their outputs
 must equal the reference C's bits exactly:
@@ -19,9 +19,10 @@ pytestmark = pytest.mark.gpu
 from oracle import Oracle  # noqa: E402
 from tupan_amd import _lib, kernels, spec  # noqa: E402
 
-IEEE_KERNELS = ("phi", "acc", "acc_jerk", "snap_crackle")
+IEEE_KERNELS = ("phi", "acc", "acc_jerk", "snap_crackle", "tstep")
 FIELDS = {"phi": spec.F5, "acc": spec.F5, "acc_jerk": spec.F8,
-          "snap_crackle": spec.F14}
+          "snap_crackle": spec.F14, "tstep": spec.F8}
+ARGS = {"tstep": (0.03125,)}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -48,7 +49,8 @@ def test_golden_bitwise(golden):
         if case["prec"] != "float64" or case["kernel"] not in IEEE_KERNELS:
             continue
         ip, jp = golden.inputs(case)
-        got = kernels.compute(case["kernel"], ip, jp, mode="fp64_ieee")
+        got = kernels.compute(case["kernel"], ip, jp, *golden.args(case),
+                              mode="fp64_ieee")
         for g, r in zip(got, golden.outputs(case)):
             assert same_bits(g, r), case
         n += 1
@@ -77,8 +79,9 @@ def test_fresh_inputs_bitwise_vs_checker(name):
                    (2113, 1), (2113, 2113), (128, 0), (129, 127)):
         ip = {f: d[f][:ni] for f in FIELDS[name]}
         jp = {f: d[f][2113 - nj:] for f in FIELDS[name]}
-        got = kernels.compute(name, ip, jp, mode="fp64_ieee")
-        ref = o.call(name + "_kernel", ip, jp)
+        args = ARGS.get(name, ())
+        got = kernels.compute(name, ip, jp, *args, mode="fp64_ieee")
+        ref = o.call(name + "_kernel", ip, jp, *args)
         for g, r in zip(got, ref):
             assert same_bits(g, r), (name, ni, nj)
 
@@ -108,7 +111,8 @@ def test_dropin_default_mode_reproduces_reference_bits(golden):
                     or case["kernel"] not in IEEE_KERNELS):
                 continue
             ip, jp = golden.inputs(case)
-            got = _call_dropin(lib, case["kernel"], ip, jp, (), np.float64)
+            got = _call_dropin(lib, case["kernel"], ip, jp, golden.args(case),
+                               np.float64)
             for g, r in zip(got, golden.outputs(case)):
                 assert same_bits(g, r), case
             n += 1

@@ -202,6 +202,54 @@ def test_trajectory_matches_reference_kernels(torch, name):
         assert (np.abs(pgot - pref) / pscale).max() < 1e-10
 
 
+IEEE_CASES = ("c1_hermite4", "ahermite4", "hermite2", "hermite6", "hermite8",
+              "sia21s_kdk", "sia21a_dkd")
+
+
+@pytest.mark.parametrize("name", IEEE_CASES)
+def test_trajectory_bitwise_in_ieee_mode(torch, name):
+    """With the pair kernels in the reference's arithmetic (fp64_ieee) and
+    the update kernels evaluating the reference's numpy expressions, a run
+    reproduces the reference-kernel trajectory bit for bit: every step
+    length, every recorded state, and (C1) every step's Tstep outputs.
+    (Cases whose kernels have no bit-exact mode -- Sakura, PNAcc -- are
+    covered to round-off above.)"""
+    from tupan_amd import integrator as ig
+    z = np.load(TRAJ, allow_pickle=False)
+    meta = [c for c in json.loads(str(z["manifest"]))["cases"]
+            if c["name"] == name][0]
+    assert not meta["pn"]
+    print("[traj-ieee] %s n=%d method=%s" % (name, meta["n"], meta["method"]),
+          flush=True)
+    init = z[name + ":init"]
+    ps = ig.System.from_host({f: init[k] for k, f in enumerate(BASE)})
+    integ = ig.Integrator(meta["eta"], 0.0, ps, meta["method"],
+                          mode="fp64_ieee")
+    keep = set(int(k) for k in z[name + ":state_steps"])
+    taus, states, tsteps = [], [], []
+
+    def record(p):
+        taus.append(float(p.tstep[0]))
+        if len(taus) - 1 in keep:
+            states.append(np.stack([getattr(p, f).cpu().numpy()
+                                    for f in STATE]))
+        if name == "c1_hermite4":
+            p.set_tstep(1.0 / 64)
+            tsteps.append(np.stack([p.tstep.cpu().numpy(),
+                                    p.tstepij.cpu().numpy()]))
+            p.tstep.fill_(taus[-1])
+    integ.evolve(meta["t_end"], callback=record)
+    assert np.array_equal(np.array(taus), z[name + ":tau"]), name
+    got, ref = np.stack(states), z[name + ":state"]
+    diff = int(np.sum(got.view(np.uint64) != ref.view(np.uint64)))
+    print("[traj-ieee] %s: %d of %d state values differ" % (name, diff,
+                                                            got.size),
+          flush=True)
+    assert diff == 0, name
+    if tsteps:
+        assert np.array_equal(np.stack(tsteps), z[name + ":tstep"])
+
+
 def test_hybrid_pn_sakura_matches_checker(torch):
     """Compact PN cluster (N=64, masses U[1,30] normalised, half-mass
     radius 0.1, unsoftened, pn_order 5 at c = 100) with close pairs

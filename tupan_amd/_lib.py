@@ -82,9 +82,11 @@ def lib():
         for name in ("phi", "acc", "acc_jerk", "snap_crackle"):
             getattr(L, "tupan_%s_host" % name).argtypes = [
                 ctypes.c_int, _SZ, _PP, _SZ, _PP, _PP]
-        L.tupan_tstep_dev.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_double,
+        L.tupan_tstep_dev.argtypes = [ctypes.c_int, _SZ, _PP, _SZ, _PP,
+                                      ctypes.c_double,
                                       _PP, _P]
-        L.tupan_tstep_host.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_double,
+        L.tupan_tstep_host.argtypes = [ctypes.c_int, _SZ, _PP, _SZ, _PP,
+                                       ctypes.c_double,
                                        _PP]
         if hasattr(L, "tupan_pnacc_dev"):
             L.tupan_pnacc_dev.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_uint,

@@ -517,22 +517,24 @@ int tupan_snap_crackle_host(int mode, size_t ni, const double *const *ip,
       TUPAN_K_SNAP_CRACKLE, mode, 14, 6, ni, ip, nj, jp, op);
 }
 
-int tupan_tstep_dev(size_t ni, const double *const *ip, size_t nj,
-                    const double *const *jp, double eta, double *const *op,
-                    void *stream) {
+int tupan_tstep_dev(int mode, size_t ni, const double *const *ip,
+                    size_t nj, const double *const *jp, double eta,
+                    double *const *op, void *stream) {
+  if (mode == TUPAN_MODE_FP64_IEEE)
+    return run_ieee_tstep(ni, ip, nj, jp, eta, op, as_stream(stream));
+  if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
+    return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
   TstepF64::Params prm{eta};
   return run_engine<TstepF64>(TUPAN_K_TSTEP, TUPAN_MODE_FP64, ni, ip, nj, jp,
                               op, prm, as_stream(stream));
 }
-int tupan_tstep_host(size_t ni, const double *const *ip, size_t nj,
+int tupan_tstep_host(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double eta, double *const *op) {
   return host_call(ni, ip, nj, jp, 8, op, 2,
                    [&](const double *const *di, const double *const *dj,
                        double *const *dout, hipStream_t st) {
-                     TstepF64::Params prm{eta};
-                     return run_engine<TstepF64>(TUPAN_K_TSTEP,
-                                                 TUPAN_MODE_FP64, ni, di, nj,
-                                                 dj, dout, prm, st);
+                     return tupan_tstep_dev(mode, ni, di, nj, dj, eta, dout,
+                                            st);
                    });
 }
 

@@ -1,13 +1,13 @@
 // ieee.hip -- the reference's arithmetic, bit for bit (TUPAN_MODE_FP64_IEEE).
 //
-// Phi, Acc, AccJerk and SnapCrackle evaluated exactly as the reference's C
-// kernels evaluate them: the same operations in the same order, one IEEE
-// operation at a time (no FMA contraction), the softened inverse square as
-// a correctly rounded division 1 / (r^2 + e2) and its square root as a
-// correctly rounded sqrt (smoothing.h:382-414), and every j subtracted from
-// one accumulator per i in ascending order.  On the same inputs the results
-// are the reference C's bits (tupan/lib/src/*_kernel_common.h, compiled as
-// ISO C99 without contraction).
+// Phi, Acc, AccJerk, SnapCrackle and Tstep evaluated exactly as the
+// reference's C kernels evaluate them: the same operations in the same
+// order, one IEEE operation at a time (no FMA contraction), the softened
+// inverse square as a correctly rounded division 1 / (r^2 + e2) and square
+// roots correctly rounded (smoothing.h:382-414), and every j folded into one
+// accumulator per i in ascending order.  On the same inputs the results are
+// the reference C's bits (tupan/lib/src/*_kernel_common.h and
+// tstep_kernel.c, compiled as ISO C99 without contraction).
 //
 // Engine.  The sequential j sum leaves one independent chain per (i,
 // output), so at N = 65536 the chip holds one wave per SIMD; the engine
@@ -25,10 +25,28 @@ namespace {
 constexpr int kIeeeTile = 64;  // j records per LDS tile
 
 // A policy supplies JRec (a multiple of 16 bytes, read from LDS as 16-byte
-// chunks), I, pack, load_i and terms(q, r, t): the kOut per-pair terms t
-// that the reference subtracts from its sums (sum -= t).
+// chunks), I, pack, load_i and terms(q, r, t, prm): the kTerms per-pair
+// terms t; IeeeBase's fold and finish are the reference's sum -= t and
+// out = sum (a policy with another accumulation overrides them).
+template <int N>
 struct IeeeBase {
   static constexpr bool kTwoLevel = false;
+  static constexpr int kOut = N, kTerms = N, kAcc = N;
+  struct Params {};
+  __device__ static void init(double *acc) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) acc[c] = 0.0;
+  }
+  __device__ static void fold(double *acc, const double *t) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int c = 0; c < N; ++c) acc[c] = acc[c] - t[c];
+  }
+  __device__ static void finish(const double *acc, Out out, int i,
+                                const Params &) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) out.f[c][i] = acc[c];
+  }
 };
 
 // smoothing.h:382-390 / :405-414 -- 1/(r^2+e2), zero unless r^2 > 0
@@ -38,8 +56,8 @@ __device__ __forceinline__ double ieee_inv_r2(double r2, double e2) {
   return r2 > 0 ? inv : 0.0;
 }
 
-struct PhiIeee : IeeeBase {
-  static constexpr int kOut = 1, kFields = 5;
+struct PhiIeee : IeeeBase<1> {
+  static constexpr int kFields = 5;
   struct JRec { double m, x, y, z, e2, pad; };
   struct I { double x, y, z, e2; };
   __device__ static JRec pack(const Soa &s, int j, double) {
@@ -49,7 +67,8 @@ struct PhiIeee : IeeeBase {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i]};
   }
   // phi_kernel_common.h:7-31
-  __device__ static void terms(const I &q, const JRec &r, double *t) {
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -61,8 +80,8 @@ struct PhiIeee : IeeeBase {
   }
 };
 
-struct AccIeee : IeeeBase {
-  static constexpr int kOut = 3, kFields = 5;
+struct AccIeee : IeeeBase<3> {
+  static constexpr int kFields = 5;
   using JRec = PhiIeee::JRec;
   using I = PhiIeee::I;
   __device__ static JRec pack(const Soa &s, int j, double c) {
@@ -72,7 +91,8 @@ struct AccIeee : IeeeBase {
     return PhiIeee::load_i(s, i);
   }
   // acc_kernel_common.h:7-37
-  __device__ static void terms(const I &q, const JRec &r, double *t) {
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -88,8 +108,8 @@ struct AccIeee : IeeeBase {
   }
 };
 
-struct AccJerkIeee : IeeeBase {
-  static constexpr int kOut = 6, kFields = 8;
+struct AccJerkIeee : IeeeBase<6> {
+  static constexpr int kFields = 8;
   struct JRec { double m, x, y, z, e2, vx, vy, vz; };
   struct I { double x, y, z, e2, vx, vy, vz; };
   __device__ static JRec pack(const Soa &s, int j, double) {
@@ -101,7 +121,8 @@ struct AccJerkIeee : IeeeBase {
             s.f[VX][i], s.f[VY][i], s.f[VZ][i]};
   }
   // acc_jerk_kernel_common.h:7-55
-  __device__ static void terms(const I &q, const JRec &r, double *t) {
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -128,8 +149,8 @@ struct AccJerkIeee : IeeeBase {
   }
 };
 
-struct SnapCrackleIeee : IeeeBase {
-  static constexpr int kOut = 6, kFields = 14;
+struct SnapCrackleIeee : IeeeBase<6> {
+  static constexpr int kFields = 14;
   struct JRec { double m, x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
   struct I { double x, y, z, e2, vx, vy, vz, ax, ay, az, jx, jy, jz; };
   __device__ static JRec pack(const Soa &s, int j, double) {
@@ -143,7 +164,8 @@ struct SnapCrackleIeee : IeeeBase {
             s.f[JX][i], s.f[JY][i], s.f[JZ][i]};
   }
   // snap_crackle_kernel_common.h:7-94
-  __device__ static void terms(const I &q, const JRec &r, double *t) {
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &) {
 #pragma clang fp contract(off)
     double rx = q.x - r.x;
     double ry = q.y - r.y;
@@ -194,12 +216,63 @@ struct SnapCrackleIeee : IeeeBase {
   }
 };
 
-template <int N>
-__device__ __forceinline__ void subtract_terms(double *acc, const double *t) {
+// tstep_kernel_common.h:7-56 and the finalisation of tstep_kernel.c:46-47:
+// per pair w^2 (masked to 0 unless r^2 > 0), summed and maximised (fmax)
+// per i; outputs eta / sqrt(1 + sum) and eta / sqrt(1 + max).
+struct TstepIeee {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kOut = 2, kTerms = 1, kAcc = 2, kFields = 8;
+  struct Params { double eta; };
+  using JRec = AccJerkIeee::JRec;
+  struct I { double m, x, y, z, e2, vx, vy, vz; };
+  __device__ static JRec pack(const Soa &s, int j, double c) {
+    return AccJerkIeee::pack(s, j, c);
+  }
+  __device__ static I load_i(const Soa &s, int i) {
+    return {s.f[M][i],  s.f[RX][i], s.f[RY][i], s.f[RZ][i],
+            s.f[E2][i], s.f[VX][i], s.f[VY][i], s.f[VZ][i]};
+  }
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &prm) {
 #pragma clang fp contract(off)
-#pragma unroll
-  for (int c = 0; c < N; ++c) acc[c] = acc[c] - t[c];
-}
+    double rx = q.x - r.x;
+    double ry = q.y - r.y;
+    double rz = q.z - r.z;
+    double e2 = q.e2 + r.e2;
+    double vx = q.vx - r.vx;
+    double vy = q.vy - r.vy;
+    double vz = q.vz - r.vz;
+    double m = q.m + r.m;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double rv = rx * vx + ry * vy + rz * vz;
+    double v2 = vx * vx + vy * vy + vz * vz;
+    bool pair = r2 > 0;
+    double inv_r2 = ieee_inv_r2(r2, e2);
+    double inv_r1 = sqrt(inv_r2);
+    double a = 2;
+    double b = (1 + a / 2) * inv_r2;
+    double w2 = inv_r2;
+    double gam = inv_r2;
+    double phi = m * inv_r1;
+    w2 *= (v2 + a * phi);
+    gam *= (w2 + b * phi);
+    gam *= (prm.eta / sqrt(w2));
+    w2 -= gam * rv;
+    t[0] = pair ? w2 : 0.0;  // a masked pair may hold 0*inf = NaN
+  }
+  __device__ static void init(double *acc) { acc[0] = acc[1] = 0.0; }
+  __device__ static void fold(double *acc, const double *t) {
+#pragma clang fp contract(off)
+    acc[0] = acc[0] + t[0];
+    acc[1] = fmax(t[0], acc[1]);
+  }
+  __device__ static void finish(const double *acc, Out out, int i,
+                                const Params &prm) {
+#pragma clang fp contract(off)
+    out.f[0][i] = prm.eta / sqrt(1 + acc[0]);
+    out.f[1][i] = prm.eta / sqrt(1 + acc[1]);
+  }
+};
 
 // One wave per 64 i-particles (one per lane); blocks of 4 waves share the
 // j tiles.  Every thread of the block takes part in every tile copy and
@@ -207,7 +280,7 @@ __device__ __forceinline__ void subtract_terms(double *acc, const double *t) {
 template <class Pol, int U>
 __global__ __launch_bounds__(256) void ieee_engine(
     Soa is, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
-    Out out) {
+    Out out, typename Pol::Params prm) {
   using JRec = typename Pol::JRec;
   static_assert(sizeof(JRec) % 16 == 0, "records are copied in 16 B chunks");
   constexpr int kRecChunks = (int)(sizeof(JRec) / 16);
@@ -219,9 +292,8 @@ __global__ __launch_bounds__(256) void ieee_engine(
   const bool active = i0 < ni;  // wave-uniform
   const int i = min(i0 + lane, ni - 1);
   const typename Pol::I q = Pol::load_i(is, i);
-  double acc[Pol::kOut];
-#pragma unroll
-  for (int c = 0; c < Pol::kOut; ++c) acc[c] = 0.0;
+  double acc[Pol::kAcc];
+  Pol::init(acc);
 
   const int ntiles = (nj + kIeeeTile - 1) / kIeeeTile;
   double2 buf[kPer];
@@ -257,31 +329,28 @@ __global__ __launch_bounds__(256) void ieee_engine(
       const int nrec = min(kIeeeTile, nj - t * kIeeeTile);
       int k = 0;
       for (; k + U <= nrec; k += U) {
-        double tt[U][Pol::kOut];
+        double tt[U][Pol::kTerms];
 #pragma unroll
-        for (int u = 0; u < U; ++u) Pol::terms(q, L[k + u], tt[u]);
+        for (int u = 0; u < U; ++u) Pol::terms(q, L[k + u], tt[u], prm);
 #pragma unroll
-        for (int u = 0; u < U; ++u) subtract_terms<Pol::kOut>(acc, tt[u]);
+        for (int u = 0; u < U; ++u) Pol::fold(acc, tt[u]);
       }
       for (; k < nrec; ++k) {
-        double t1[Pol::kOut];
-        Pol::terms(q, L[k], t1);
-        subtract_terms<Pol::kOut>(acc, t1);
+        double t1[Pol::kTerms];
+        Pol::terms(q, L[k], t1, prm);
+        Pol::fold(acc, t1);
       }
     }
     if (t + 1 < ntiles) store((t + 1) & 1, t + 1);
     __syncthreads();
   }
-  if (active && i0 + lane < ni) {
-#pragma unroll
-    for (int c = 0; c < Pol::kOut; ++c) out.f[c][i0 + lane] = acc[c];
-  }
+  if (active && i0 + lane < ni) Pol::finish(acc, out, i0 + lane, prm);
 }
 
 template <class Pol, int U>
 int run_ieee_engine(size_t ni, const double *const *ip, size_t nj,
                     const double *const *jp, double *const *op,
-                    hipStream_t st) {
+                    hipStream_t st, typename Pol::Params prm = {}) {
   if (ni == 0) return TUPAN_OK;
   if (!ip || !jp || !op) return set_error(TUPAN_ERR_ARG, "NULL field array");
   if (ni > (size_t)INT_MAX / 8 || nj > (size_t)INT_MAX / 8)
@@ -313,7 +382,7 @@ int run_ieee_engine(size_t ni, const double *const *ip, size_t nj,
   if (timed) (void)hipEventRecord(ev0, st);
   hipLaunchKernelGGL((ieee_engine<Pol, U>), dim3((n_i + 255) / 256),
                      dim3(256), 0, st, is, n_i,
-                     (const typename Pol::JRec *)jr, n_j, out);
+                     (const typename Pol::JRec *)jr, n_j, out, prm);
   if (timed) (void)hipEventRecord(ev1, st);
   hipError_t e = hipGetLastError();
   device_free(jr, st);
@@ -338,6 +407,13 @@ int run_ieee(int kid, size_t ni, const double *const *ip, size_t nj,
       return run_ieee_engine<SnapCrackleIeee, 2>(ni, ip, nj, jp, op, st);
   }
   return set_error(TUPAN_ERR_ARG, "no IEEE mode for kernel %d", kid);
+}
+
+int run_ieee_tstep(size_t ni, const double *const *ip, size_t nj,
+                   const double *const *jp, double eta, double *const *op,
+                   hipStream_t st) {
+  return run_ieee_engine<TstepIeee, 4>(ni, ip, nj, jp, op, st,
+                                       TstepIeee::Params{eta});
 }
 
 }  // namespace tupan

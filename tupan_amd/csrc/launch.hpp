@@ -35,9 +35,12 @@ int host_call(size_t ni, const double *const *ip, size_t nj,
               const double *const *jp, int nfields, double *const *op,
               int nout, const DevFn &fn);
 
-// ---- ieee.hip: TUPAN_MODE_FP64_IEEE for Phi, Acc, AccJerk, SnapCrackle ----
+// ---- ieee.hip: TUPAN_MODE_FP64_IEEE (Phi, Acc, AccJerk, SnapCrackle, Tstep) ----
 int run_ieee(int kid, size_t ni, const double *const *ip, size_t nj,
              const double *const *jp, double *const *op, hipStream_t st);
+int run_ieee_tstep(size_t ni, const double *const *ip, size_t nj,
+                   const double *const *jp, double eta, double *const *op,
+                   hipStream_t st);
 
 // ---- generic engine launch ----
 double near_field_factor();

@@ -521,8 +521,9 @@ class NREG(Base):
         super(NREG, self).evolve_step(t_end)
 
     def nreg_x(self, ps, dt):
-        """nreg.py:22-33."""
-        mtot = ps.total_mass()
+        """nreg.py:22-33 (the total mass is taken once per step: masses do
+        not change within one)."""
+        mtot = self._mtot
         ps._pair("nreg_x", dt)
         _lib.call_update("divide", ps.n, 3, ps.ptrs(("mrx", "mry", "mrz")),
                          ps.ptrs(("rx", "ry", "rz")), float(mtot), _stream(ps))
@@ -531,7 +532,7 @@ class NREG(Base):
 
     def nreg_v(self, ps, dt):
         """nreg.py:45-59."""
-        mtot = ps.total_mass()
+        mtot = self._mtot
         ps._pair("nreg_v", dt)
         _lib.call_update("divide", ps.n, 3, ps.ptrs(("mvx", "mvy", "mvz")),
                          ps.ptrs(("vx", "vy", "vz")), float(mtot), _stream(ps))
@@ -552,6 +553,7 @@ class NREG(Base):
 
     def do_step(self, ps, tau):
         """nreg.py:149-163."""
+        self._mtot = ps.total_mass()
         t0 = ps.t_curr
         if "anreg" in self.method:
             self.anreg_step(ps, tau / 2)

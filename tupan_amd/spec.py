@@ -24,7 +24,7 @@ KERNELS = {
                        ("ax", "ay", "az", "jx", "jy", "jz"), True, 42),
     "snap_crackle": Kernel("snap_crackle", 3, F14,
                            ("sx", "sy", "sz", "cx", "cy", "cz"), True, 114),
-    "tstep": Kernel("tstep", 4, F8, ("tstep", "tstepij"), False, 42),
+    "tstep": Kernel("tstep", 4, F8, ("tstep", "tstepij"), True, 42),
     "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), False, None),
     "sakura": Kernel("sakura", 6, F8,
                      ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
