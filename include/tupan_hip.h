@@ -33,9 +33,9 @@
  *                               contraction, correctly rounded division and
  *                               sqrt, j summed in ascending order (one j
  *                               split); results equal the reference's bits
- *       (modes apply to Phi, Acc, AccJerk, SnapCrackle and Tstep -- which
- *       has no FP32C variant and runs FP64 there; PNAcc, Sakura and NREG
- *       always run FP64 and take no mode).
+ *       (Phi, Acc, AccJerk and SnapCrackle have all three; Tstep, PNAcc and
+ *       NREG X/V have no FP32C variant and run FP64 there; Sakura and
+ *       Kepler always run FP64 and take no mode).
  *
  * Every function returns 0 on success or a TUPAN_ERR_* code; the text of the
  * last error on the calling thread is tupan_last_error().
@@ -143,11 +143,12 @@ int tupan_tstep_host(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double eta, double *const *op);
 
 /* ---- PNAcc: replaces pnacc_kernel, libtupan.h:121-150 ----------------
- * ip, jp: F8.  inv[0..6] = 1/c^1 .. 1/c^7.  op: pnax, pnay, pnaz. */
-int tupan_pnacc_dev(size_t ni, const double *const *ip, size_t nj,
+ * ip, jp: F8.  inv[0..6] = 1/c^1 .. 1/c^7.  op: pnax, pnay, pnaz.
+ * mode: FP64 (FP32C runs FP64) or FP64_IEEE. */
+int tupan_pnacc_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                     const double *const *jp, unsigned order,
                     const double *inv, double *const *op, void *stream);
-int tupan_pnacc_host(size_t ni, const double *const *ip, size_t nj,
+int tupan_pnacc_host(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, unsigned order,
                      const double *inv, double *const *op);
 
@@ -166,21 +167,24 @@ int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,
 /* ---- NREG X: replaces nreg_Xkernel, libtupan.h:152-179 ---------------
  * ip, jp: F8.  op: mrx, mry, mrz, ax, ay, az, u -- sums over j at the drifted
  * separation r_ij + v_ij dt of m_j r, the softened acceleration, and
- * u_i = m_i sum_j m_j / s (FP64). */
-int tupan_nreg_x_dev(size_t ni, const double *const *ip, size_t nj,
+ * u_i = m_i sum_j m_j / s.  mode: FP64 (FP32C runs FP64) or FP64_IEEE. */
+int tupan_nreg_x_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double dt, double *const *op,
                      void *stream);
-int tupan_nreg_x_host(size_t ni, const double *const *ip, size_t nj,
-                      const double *const *jp, double dt, double *const *op);
+int tupan_nreg_x_host(int mode, size_t ni, const double *const *ip,
+                      size_t nj, const double *const *jp, double dt,
+                      double *const *op);
 
 /* ---- NREG V: replaces nreg_Vkernel, libtupan.h:181-201 ---------------
  * ip, jp: m, vx, vy, vz, ax, ay, az.  op: mvx, mvy, mvz, mk -- sums over j
- * of m_j w and k_i = m_i sum_j m_j |w|^2 with w = v_ij + a_ij dt (FP64). */
-int tupan_nreg_v_dev(size_t ni, const double *const *ip, size_t nj,
+ * of m_j w and k_i = m_i sum_j m_j |w|^2 with w = v_ij + a_ij dt.  mode as
+ * for NREG X. */
+int tupan_nreg_v_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double dt, double *const *op,
                      void *stream);
-int tupan_nreg_v_host(size_t ni, const double *const *ip, size_t nj,
-                      const double *const *jp, double dt, double *const *op);
+int tupan_nreg_v_host(int mode, size_t ni, const double *const *ip,
+                      size_t nj, const double *const *jp, double dt,
+                      double *const *op);
 
 /* ---- Kepler: replaces kepler_solver_kernel, libtupan.h:231-246 -------
  * ip: F8 arrays of which particles 0 and 1 are used; op: rx, ry, rz, vx,

@@ -1,6 +1,7 @@
 """The reference-arithmetic mode (fp64_ieee) on the GPU: bit for bit.
 
-Phi, Acc, AccJerk, SnapCrackle and Tstep in fp64_ieee perform the reference C's
+Phi, Acc, AccJerk, SnapCrackle, Tstep, PNAcc and NREG X/V in fp64_ieee
+perform the reference C's
 operations in its order with its rounding (no FMA contraction, correctly
 rounded division and sqrt, j summed in ascending order), so their outputs
 must equal the reference C's bits exactly:
@@ -19,10 +20,18 @@ pytestmark = pytest.mark.gpu
 from oracle import Oracle  # noqa: E402
 from tupan_amd import _lib, kernels, spec  # noqa: E402
 
-IEEE_KERNELS = ("phi", "acc", "acc_jerk", "snap_crackle", "tstep")
+IEEE_KERNELS = ("phi", "acc", "acc_jerk", "snap_crackle", "tstep", "pnacc",
+                "nreg_x", "nreg_v")
+NREG_V_FIELDS = ("mass", "vx", "vy", "vz", "ax", "ay", "az")
 FIELDS = {"phi": spec.F5, "acc": spec.F5, "acc_jerk": spec.F8,
-          "snap_crackle": spec.F14, "tstep": spec.F8}
-ARGS = {"tstep": (0.03125,)}
+          "snap_crackle": spec.F14, "tstep": spec.F8, "pnacc": spec.F8,
+          "nreg_x": spec.F8, "nreg_v": NREG_V_FIELDS}
+ORACLE_NAME = {"nreg_x": "nreg_Xkernel", "nreg_v": "nreg_Vkernel"}
+# every PN order term active (7: 1PN .. 3.5PN) at c = 10
+_INV = tuple(10.0**-k for k in range(1, 8))
+ARGS = {"tstep": ((0.03125,), (0.03125,)),
+        "pnacc": ((7, _INV), (7,) + _INV),
+        "nreg_x": ((0.01,), (0.01,)), "nreg_v": ((0.01,), (0.01,))}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -79,9 +88,9 @@ def test_fresh_inputs_bitwise_vs_checker(name):
                    (2113, 1), (2113, 2113), (128, 0), (129, 127)):
         ip = {f: d[f][:ni] for f in FIELDS[name]}
         jp = {f: d[f][2113 - nj:] for f in FIELDS[name]}
-        args = ARGS.get(name, ())
+        args, oargs = ARGS.get(name, ((), ()))
         got = kernels.compute(name, ip, jp, *args, mode="fp64_ieee")
-        ref = o.call(name + "_kernel", ip, jp, *args)
+        ref = o.call(ORACLE_NAME.get(name, name + "_kernel"), ip, jp, *oargs)
         for g, r in zip(got, ref):
             assert same_bits(g, r), (name, ni, nj)
 

@@ -203,7 +203,7 @@ def test_trajectory_matches_reference_kernels(torch, name):
 
 
 IEEE_CASES = ("c1_hermite4", "ahermite4", "hermite2", "hermite6", "hermite8",
-              "sia21s_kdk", "sia21a_dkd")
+              "sia21s_kdk", "sia21a_dkd", "sia21s_kdk_pn")
 
 
 @pytest.mark.parametrize("name", IEEE_CASES)
@@ -211,20 +211,22 @@ def test_trajectory_bitwise_in_ieee_mode(torch, name):
     """With the pair kernels in the reference's arithmetic (fp64_ieee) and
     the update kernels evaluating the reference's numpy expressions, a run
     reproduces the reference-kernel trajectory bit for bit: every step
-    length, every recorded state, and (C1) every step's Tstep outputs.
-    (Cases whose kernels have no bit-exact mode -- Sakura, PNAcc -- are
-    covered to round-off above.)"""
+    length, every recorded state, (C1) every step's Tstep outputs and (PN)
+    the PN accumulators.  (Sakura, whose Kepler solve has no bit-exact
+    mode, is covered to round-off above.)"""
     from tupan_amd import integrator as ig
     z = np.load(TRAJ, allow_pickle=False)
     meta = [c for c in json.loads(str(z["manifest"]))["cases"]
             if c["name"] == name][0]
-    assert not meta["pn"]
     print("[traj-ieee] %s n=%d method=%s" % (name, meta["n"], meta["method"]),
           flush=True)
     init = z[name + ":init"]
     ps = ig.System.from_host({f: init[k] for k, f in enumerate(BASE)})
+    kw = {}
+    if meta["pn"]:
+        kw = dict(pn_order=meta["pn"][0], clight=meta["pn"][1])
     integ = ig.Integrator(meta["eta"], 0.0, ps, meta["method"],
-                          mode="fp64_ieee")
+                          mode="fp64_ieee", **kw)
     keep = set(int(k) for k in z[name + ":state_steps"])
     taus, states, tsteps = [], [], []
 
@@ -248,6 +250,10 @@ def test_trajectory_bitwise_in_ieee_mode(torch, name):
     assert diff == 0, name
     if tsteps:
         assert np.array_equal(np.stack(tsteps), z[name + ":tstep"])
+    if meta["pn"]:
+        pgot = np.stack([ps.field(f).cpu().numpy() for f in oi.PN_FIELDS])
+        assert np.array_equal(pgot.view(np.uint64),
+                              z[name + ":pn"].view(np.uint64))
 
 
 def test_hybrid_pn_sakura_matches_checker(torch):

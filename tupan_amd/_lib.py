@@ -89,10 +89,10 @@ def lib():
                                        ctypes.c_double,
                                        _PP]
         if hasattr(L, "tupan_pnacc_dev"):
-            L.tupan_pnacc_dev.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_uint,
-                                          _P, _PP, _P]
-            L.tupan_pnacc_host.argtypes = [_SZ, _PP, _SZ, _PP, ctypes.c_uint,
-                                           _P, _PP]
+            L.tupan_pnacc_dev.argtypes = [ctypes.c_int, _SZ, _PP, _SZ, _PP,
+                                          ctypes.c_uint, _P, _PP, _P]
+            L.tupan_pnacc_host.argtypes = [ctypes.c_int, _SZ, _PP, _SZ, _PP,
+                                           ctypes.c_uint, _P, _PP]
         if hasattr(L, "tupan_sakura_dev"):
             L.tupan_sakura_dev.argtypes = [_SZ, _PP, _SZ, _PP,
                                            ctypes.c_double, ctypes.c_int,
@@ -114,9 +114,9 @@ def lib():
         L.tupan_diagnostics_dev.argtypes = [_SZ, _PP, ctypes.c_int, _P, _P]
         for name in ("nreg_x", "nreg_v"):
             getattr(L, "tupan_%s_dev" % name).argtypes = [
-                _SZ, _PP, _SZ, _PP, _D, _PP, _P]
+                _I, _SZ, _PP, _SZ, _PP, _D, _PP, _P]
             getattr(L, "tupan_%s_host" % name).argtypes = [
-                _SZ, _PP, _SZ, _PP, _D, _PP]
+                _I, _SZ, _PP, _SZ, _PP, _D, _PP]
         L.tupan_kepler_dev.argtypes = [_PP, _D, _PP, _P, _P]
         L.tupan_kepler_host.argtypes = [_PP, _D, _PP]
         L.tupan_acc_far_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _PP, _P]

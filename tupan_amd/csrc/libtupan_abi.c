@@ -105,10 +105,10 @@ static void forward(const char *who, int kind, mode_fn fn, size_t ni,
         switch (kind) {
         case 0: rc = fn(mode, ni, iw, nj, jw, ow); break;
         case 1: rc = tupan_tstep_host(mode, ni, iw, nj, jw, eta, ow); break;
-        case 2: rc = tupan_pnacc_host(ni, iw, nj, jw, order, inv, ow); break;
+        case 2: rc = tupan_pnacc_host(mode, ni, iw, nj, jw, order, inv, ow); break;
         case 3: rc = tupan_sakura_host(ni, iw, nj, jw, dt, flag, ow); break;
-        case 4: rc = tupan_nreg_x_host(ni, iw, nj, jw, dt, ow); break;
-        case 5: rc = tupan_nreg_v_host(ni, iw, nj, jw, dt, ow); break;
+        case 4: rc = tupan_nreg_x_host(mode, ni, iw, nj, jw, dt, ow); break;
+        case 5: rc = tupan_nreg_v_host(mode, ni, iw, nj, jw, dt, ow); break;
         }
     }
     for (int k = 0; k < nout; ++k) {

@@ -11,6 +11,7 @@
 //           kepler_solver_kernel.c:5-49): particles 0 and 1 advanced by dt
 //           along their softened Kepler orbit about the freely drifting
 //           centre of mass (twobody.hpp's kepler_drift).
+#include "ieee_engine.hpp"
 #include "launch.hpp"
 #include "twobody.hpp"
 
@@ -107,6 +108,103 @@ struct NregVF64 {
   }
 };
 
+// The same sums in the reference's arithmetic (TUPAN_MODE_FP64_IEEE): its
+// operations in its order without contraction, correctly rounded division
+// and sqrt, one accumulator per i in ascending j order.
+struct NregXIeee : ieee::IeeeBase<7> {
+  static constexpr int kFields = 8;
+  using Params = NregParams;
+  using JRec = NregXF64::JRec;
+  using I = JRec;
+  __device__ static JRec pack(const Soa &s, int j, double c) {
+    return NregXF64::pack(s, j, c);
+  }
+  __device__ static I load_i(const Soa &s, int i) {
+    return NregXF64::pack(s, i, 0.0);
+  }
+  // nreg_kernels_common.h:7-58
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &p) {
+#pragma clang fp contract(off)
+    double rx = q.x - r.x;
+    double ry = q.y - r.y;
+    double rz = q.z - r.z;
+    double e2 = q.e2 + r.e2;
+    double vx = q.vx - r.vx;
+    double vy = q.vy - r.vy;
+    double vz = q.vz - r.vz;
+    rx += vx * p.dt;
+    ry += vy * p.dt;
+    rz += vz * p.dt;
+    double r2 = rx * rx + ry * ry + rz * rz;
+    double inv_r2 = ieee::ieee_inv_r2(r2, e2);
+    double inv_r1 = sqrt(inv_r2);
+    double inv_r3 = inv_r2 * inv_r1;
+    inv_r3 *= r.m;
+    t[0] = r.m * rx;
+    t[1] = r.m * ry;
+    t[2] = r.m * rz;
+    t[3] = inv_r3 * rx;
+    t[4] = inv_r3 * ry;
+    t[5] = inv_r3 * rz;
+    t[6] = r.m * inv_r1;
+  }
+  __device__ static void fold(double *acc, const double *t) {
+#pragma clang fp contract(off)
+    for (int c = 0; c < 3; ++c) acc[c] = acc[c] + t[c];
+    for (int c = 3; c < 6; ++c) acc[c] = acc[c] - t[c];
+    acc[6] = acc[6] + t[6];
+  }
+  __device__ static void finish(const double *acc, Out out, int i,
+                                const Params &p) {
+#pragma clang fp contract(off)
+    for (int c = 0; c < 6; ++c) out.f[c][i] = acc[c];
+    out.f[6][i] = p.im[i] * acc[6];
+  }
+};
+
+struct NregVIeee : ieee::IeeeBase<4> {
+  static constexpr int kFields = 7;
+  using Params = NregParams;
+  using JRec = NregVF64::JRec;
+  using I = JRec;
+  __device__ static JRec pack(const Soa &s, int j, double c) {
+    return NregVF64::pack(s, j, c);
+  }
+  __device__ static I load_i(const Soa &s, int i) {
+    return NregVF64::pack(s, i, 0.0);
+  }
+  // nreg_kernels_common.h:61-99
+  __device__ static void terms(const I &q, const JRec &r, double *t,
+                               const Params &p) {
+#pragma clang fp contract(off)
+    double vx = q.vx - r.vx;
+    double vy = q.vy - r.vy;
+    double vz = q.vz - r.vz;
+    double ax = q.ax - r.ax;
+    double ay = q.ay - r.ay;
+    double az = q.az - r.az;
+    vx += ax * p.dt;
+    vy += ay * p.dt;
+    vz += az * p.dt;
+    double v2 = vx * vx + vy * vy + vz * vz;
+    t[0] = r.m * vx;
+    t[1] = r.m * vy;
+    t[2] = r.m * vz;
+    t[3] = r.m * v2;
+  }
+  __device__ static void fold(double *acc, const double *t) {
+#pragma clang fp contract(off)
+    for (int c = 0; c < 4; ++c) acc[c] = acc[c] + t[c];
+  }
+  __device__ static void finish(const double *acc, Out out, int i,
+                                const Params &p) {
+#pragma clang fp contract(off)
+    for (int c = 0; c < 3; ++c) out.f[c][i] = acc[c];
+    out.f[3][i] = p.im[i] * acc[3];
+  }
+};
+
 // One thread: reads particles 0 and 1 completely before writing, so the
 // outputs may alias the inputs (the reference updates ips in place,
 // tupan/lib/extensions.py:642-646).
@@ -147,43 +245,60 @@ using namespace tupan;
 
 extern "C" {
 
-int tupan_nreg_x_dev(size_t ni, const double *const *ip, size_t nj,
+static int nreg_mode_ok(int mode) {
+  if (mode == TUPAN_MODE_FP64 || mode == TUPAN_MODE_FP32C ||
+      mode == TUPAN_MODE_FP64_IEEE)
+    return TUPAN_OK;
+  return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
+}
+
+int tupan_nreg_x_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double dt, double *const *op,
                      void *stream) {
+  if (int rc = nreg_mode_ok(mode)) return rc;
   if (ni && (!ip || !ip[M]))
     return set_error(TUPAN_ERR_ARG, "NULL i mass field");
   NregParams prm{dt, ni ? ip[M] : nullptr};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (mode == TUPAN_MODE_FP64_IEEE)
+    return ieee::run_ieee_engine<NregXIeee, 4>(ni, ip, nj, jp, op, st, prm);
   return run_engine<NregXF64>(TUPAN_K_NREG_X, TUPAN_MODE_FP64, ni, ip, nj,
-                              jp, op, prm,
-                              reinterpret_cast<hipStream_t>(stream));
+                              jp, op, prm, st);
 }
 
-int tupan_nreg_x_host(size_t ni, const double *const *ip, size_t nj,
-                      const double *const *jp, double dt, double *const *op) {
+int tupan_nreg_x_host(int mode, size_t ni, const double *const *ip,
+                      size_t nj, const double *const *jp, double dt,
+                      double *const *op) {
   return host_call(ni, ip, nj, jp, 8, op, 7,
                    [&](const double *const *di, const double *const *dj,
                        double *const *dout, hipStream_t st) {
-                     return tupan_nreg_x_dev(ni, di, nj, dj, dt, dout, st);
+                     return tupan_nreg_x_dev(mode, ni, di, nj, dj, dt, dout,
+                                             st);
                    });
 }
 
-int tupan_nreg_v_dev(size_t ni, const double *const *ip, size_t nj,
+int tupan_nreg_v_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double dt, double *const *op,
                      void *stream) {
+  if (int rc = nreg_mode_ok(mode)) return rc;
   if (ni && (!ip || !ip[0]))
     return set_error(TUPAN_ERR_ARG, "NULL i mass field");
   NregParams prm{dt, ni ? ip[0] : nullptr};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (mode == TUPAN_MODE_FP64_IEEE)
+    return ieee::run_ieee_engine<NregVIeee, 4>(ni, ip, nj, jp, op, st, prm);
   return run_engine<NregVF64>(TUPAN_K_NREG_V, TUPAN_MODE_FP64, ni, ip, nj,
-                              jp, op, prm,
-                              reinterpret_cast<hipStream_t>(stream));
+                              jp, op, prm, st);
 }
 
-int tupan_nreg_v_host(size_t ni, const double *const *ip, size_t nj,
-                      const double *const *jp, double dt, double *const *op) {
+int tupan_nreg_v_host(int mode, size_t ni, const double *const *ip,
+                      size_t nj, const double *const *jp, double dt,
+                      double *const *op) {
   return host_call(ni, ip, nj, jp, 7, op, 4,
                    [&](const double *const *di, const double *const *dj,
                        double *const *dout, hipStream_t st) {
-                     return tupan_nreg_v_dev(ni, di, nj, dj, dt, dout, st);
+                     return tupan_nreg_v_dev(mode, ni, di, nj, dj, dt, dout,
+                                             st);
                    });
 }
 

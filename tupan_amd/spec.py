@@ -25,13 +25,13 @@ KERNELS = {
     "snap_crackle": Kernel("snap_crackle", 3, F14,
                            ("sx", "sy", "sz", "cx", "cy", "cz"), True, 114),
     "tstep": Kernel("tstep", 4, F8, ("tstep", "tstepij"), True, 42),
-    "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), False, None),
+    "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), True, None),
     "sakura": Kernel("sakura", 6, F8,
                      ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
     "nreg_x": Kernel("nreg_x", 9, F8,
-                     ("mrx", "mry", "mrz", "ax", "ay", "az", "u"), False, 37),
+                     ("mrx", "mry", "mrz", "ax", "ay", "az", "u"), True, 37),
     "nreg_v": Kernel("nreg_v", 10, ("mass", "vx", "vy", "vz", "ax", "ay", "az"),
-                     ("mvx", "mvy", "mvz", "mk"), False, 25),
+                     ("mvx", "mvy", "mvz", "mk"), True, 25),
     # pair-selected variants for hybrid schemes (no reference counterpart):
     # Acc over the pairs with r >= rcut, Sakura over the pairs with r < rcut
     "acc_far": Kernel("acc_far", 1, F5, ("ax", "ay", "az"), False, 20),
