@@ -334,6 +334,16 @@ int tupan_divide_dev(size_t n, int k, const double *const *src,
  *                          sum m r (+pn_mr), sum m v (+pn_mv), sum r x m v
  *                          (+pn_am). */
 int tupan_sum_dev(size_t n, const double *x, double *out, void *stream);
+/* out[0] = sum_i x[i] in numpy's order for a contiguous float64 array
+ * (0 + the pairwise sums of consecutive 8192-element chunks): the bits of
+ * the reference drivers' ps.x.sum(). */
+int tupan_sum_pairwise_dev(size_t n, const double *x, double *out,
+                           void *stream);
+/* Per-particle ke = 0.5 m (vx^2 + vy^2 + vz^2) and mphi = m phi (body.py
+ * :261-275, :294-300) from f = {m, vx, vy, vz, phi}, evaluated as numpy
+ * does, for totals summed in numpy's order. */
+int tupan_energy_terms_dev(size_t n, const double *const *f, double *ke,
+                           double *mphi, void *stream);
 int tupan_diagnostics_dev(size_t n, const double *const *f, int pn,
                           double *out, void *stream);
 

@@ -10,6 +10,8 @@
 * NREG (nreg, anreg) on a 64-body Plummer model, 6 steps, against the
   checker's NREG on the reference's C kernels: positions, velocities and
   the physical clock within 1e-11;
+* NREG in the reference-arithmetic mode: bit for bit (numpy-order sums);
+* tupan_sum_pairwise_dev: numpy's float64 sum, bit for bit;
 * FewBody: n = 1 drift and n = 2/3 Kepler solve against the checker.
 """
 import math
@@ -65,6 +67,22 @@ def test_sum_matches_exact_sum(n):
     again = torch.empty(1, dtype=torch.float64, device="cuda")
     _lib.call_update("sum", n, t.data_ptr(), again.data_ptr(), 0)
     assert float(again.item()) == got  # fixed tree: bit-reproducible
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 8, 127, 128, 129, 1000, 8191, 8192,
+                               8193, 16384, 65539, 200003])
+def test_sum_pairwise_equals_numpy_bits(n):
+    """tupan_sum_pairwise_dev reproduces numpy's float64 sum bit for bit
+    (the reference drivers' ps.x.sum()), across the leaf (128), chunk
+    (8192) and partial-chunk cases."""
+    import torch
+    rng = np.random.RandomState(n % 89)
+    x = rng.uniform(0.5, 1.5, n) * (1 + 1e-3 * rng.standard_normal(n))
+    t = torch.as_tensor(x, device="cuda")
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.call_update("sum_pairwise", n, t.data_ptr(), out.data_ptr(), 0)
+    got = float(out.item())
+    assert np.float64(got).view(np.uint64) == np.sum(x).view(np.uint64), n
 
 
 @pytest.mark.parametrize("pn", [False, True])
@@ -155,6 +173,30 @@ def test_nreg_matches_checker(K, method):
     tstep = s.tstep.cpu().numpy()
     assert np.all(tstep == tstep[0])
     assert abs(tstep[0] - host["tstep"][0]) <= 1e-11 * abs(host["tstep"][0])
+
+
+@pytest.mark.parametrize("method", ["nreg", "anreg"])
+def test_nreg_bitwise_in_ieee_mode(K, method):
+    """With the NREG and Phi kernels in the reference's arithmetic and every
+    sum in numpy's order, the device NREG reproduces the checker's NREG on
+    the reference's C kernels bit for bit: positions, velocities and the
+    physical clock after every step."""
+    from tupan_amd.integrator import Integrator
+    d = _plummer(200, 14, 1e-3)
+    s = _system(d)
+    integ = Integrator(1.0 / 32, 0.0, s, method, mode="fp64_ieee")
+    host = dict(d)
+    host["t_curr"] = 0.0
+    st = {}
+    I.nreg_initialize(K, host, st)
+    for k in range(6):
+        integ.evolve_step(10.0)
+        I.nreg_do_step(K, host, st, 1.0 / 32, method)
+        got = s.to_host(BASE)
+        for f in ("rx", "ry", "rz", "vx", "vy", "vz"):
+            assert np.array_equal(got[f].view(np.uint64),
+                                  host[f].view(np.uint64)), (k, f)
+        assert s.t_curr == host["t_curr"], k
 
 
 @pytest.mark.parametrize("n", [1, 2, 3])

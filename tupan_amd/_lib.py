@@ -111,6 +111,8 @@ def lib():
         L.tupan_divide_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP,
                                        ctypes.c_double, _P]
         L.tupan_sum_dev.argtypes = [_SZ, _P, _P, _P]
+        L.tupan_sum_pairwise_dev.argtypes = [_SZ, _P, _P, _P]
+        L.tupan_energy_terms_dev.argtypes = [_SZ, _PP, _P, _P, _P]
         L.tupan_diagnostics_dev.argtypes = [_SZ, _PP, ctypes.c_int, _P, _P]
         for name in ("nreg_x", "nreg_v"):
             getattr(L, "tupan_%s_dev" % name).argtypes = [

@@ -6,6 +6,14 @@
 //                          (tupan/simulation.py:96-129), one pass over the
 //                          particles with the per-particle expressions of
 //                          tupan/particles/body.py
+//   tupan_sum_pairwise_dev the sum in numpy's order for a contiguous float64
+//                          array (what the reference's ps.x.sum() computes):
+//                          0 + the pairwise sums of consecutive 8192-element
+//                          chunks, each split in halves rounded to multiples
+//                          of 8 down to <= 128 elements, which are summed
+//                          with 8 interleaved accumulators
+//   tupan_energy_terms_dev the per-particle 0.5 m v^2 and m phi of body.py
+//                          (:261-275, :294-300), for sums in numpy's order
 //
 // Two launches: B <= 256 blocks accumulate grid-strided per-thread sums and
 // reduce them through LDS in a fixed tree; one block adds the B partials
@@ -142,6 +150,87 @@ int run_reduce(const Fields &p, size_t n, double *out, hipStream_t st) {
   return TUPAN_OK;
 }
 
+// ---- numpy's summation order -------------------------------------------
+constexpr int kNpChunk = 8192;  // numpy's reduction buffer (elements)
+constexpr int kNpLeaf = 128;    // pairwise-sum block size
+
+// a block of n <= 128 elements (n < 8: one running sum from 0)
+__device__ double np_leaf(const double *a, int n) {
+#pragma clang fp contract(off)
+  if (n < 8) {
+    double res = 0.0;
+    for (int i = 0; i < n; ++i) res = res + a[i];
+    return res;
+  }
+  double r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = a[j];
+  int i = 8;
+  const int lim = n - (n % 8);
+  for (; i < lim; i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = r[j] + a[i + j];
+  }
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res = res + a[i];
+  return res;
+}
+
+// halves rounded down to a multiple of 8; n <= 8192 needs 7 levels
+template <int D>
+__device__ __noinline__ double np_pairwise(const double *a, int n) {
+  if (n <= kNpLeaf || D == 0) return np_leaf(a, n);
+  int n2 = n / 2;
+  n2 -= n2 % 8;
+  return np_pairwise<(D > 0 ? D - 1 : 0)>(a, n2) +
+         np_pairwise<(D > 0 ? D - 1 : 0)>(a + n2, n - n2);
+}
+
+// One wave per chunk, staged in LDS.  A full chunk splits into 64 leaves of
+// 128 that the recursion combines as a perfect binary tree: one leaf per
+// lane, then the same tree level by level.  A partial (last) chunk follows
+// the recursion on one lane.
+__global__ __launch_bounds__(64) void np_chunk_kernel(const double *x, int n,
+                                                     double *part) {
+  __shared__ double buf[kNpChunk];
+  const int lane = threadIdx.x;
+  const int base = blockIdx.x * kNpChunk;
+  const int m = min(kNpChunk, n - base);
+  for (int i = lane; i < m; i += 64) buf[i] = x[base + i];
+  __syncthreads();
+  if (m == kNpChunk) {
+    const double v = np_leaf(buf + kNpLeaf * lane, kNpLeaf);
+    __syncthreads();
+    buf[lane] = v;
+    __syncthreads();
+    for (int s = 1; s < 64; s *= 2) {
+      if (lane % (2 * s) == 0) buf[lane] = buf[lane] + buf[lane + s];
+      __syncthreads();
+    }
+    if (lane == 0) part[blockIdx.x] = buf[0];
+  } else if (lane == 0) {
+    part[blockIdx.x] = np_pairwise<10>(buf, m);
+  }
+}
+
+__global__ void np_final_kernel(const double *part, int nchunks, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double r = 0.0;
+  for (int c = 0; c < nchunks; ++c) r = r + part[c];
+  out[0] = r;
+}
+
+__global__ __launch_bounds__(256) void energy_terms_kernel(
+    int n, const double *m, const double *vx, const double *vy,
+    const double *vz, const double *phi, double *ke, double *mphi) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n;
+       i += gridDim.x * 256) {
+    ke[i] = 0.5 * m[i] * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+    mphi[i] = m[i] * phi[i];
+  }
+}
+
 }  // namespace
 }  // namespace tupan
 
@@ -154,6 +243,45 @@ int tupan_sum_dev(size_t n, const double *x, double *out, void *stream) {
   Fields p{};
   p.f[0] = x;
   return run_reduce<SumFn>(p, n, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+int tupan_sum_pairwise_dev(size_t n, const double *x, double *out,
+                           void *stream) {
+  if (n > (size_t)INT_MAX)
+    return set_error(TUPAN_ERR_ARG, "n too large (%zu)", n);
+  if ((n && !x) || !out) return set_error(TUPAN_ERR_ARG, "NULL pointer");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int nchunks = (int)((n + kNpChunk - 1) / kNpChunk);
+  double *part = nullptr;
+  if (nchunks > 0) {
+    int rc = device_alloc((void **)&part, sizeof(double) * nchunks, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(np_chunk_kernel, dim3(nchunks), dim3(64), 0, st, x,
+                       (int)n, part);
+  }
+  hipLaunchKernelGGL(np_final_kernel, dim3(1), dim3(1), 0, st,
+                     (const double *)part, nchunks, out);
+  hipError_t e = hipGetLastError();
+  device_free(part, st);
+  if (e != hipSuccess) return hip_fail(e, "pairwise sum launch");
+  return TUPAN_OK;
+}
+
+int tupan_energy_terms_dev(size_t n, const double *const *f, double *ke,
+                           double *mphi, void *stream) {
+  if (n > (size_t)INT_MAX)
+    return set_error(TUPAN_ERR_ARG, "n too large (%zu)", n);
+  if (!f || !ke || !mphi) return set_error(TUPAN_ERR_ARG, "NULL pointer");
+  for (int k = 0; k < 5; ++k)
+    if (n && !f[k]) return set_error(TUPAN_ERR_ARG, "NULL field %d", k);
+  if (n == 0) return TUPAN_OK;
+  const size_t g0 = (n + 255) / 256;
+  const int g = g0 > 65535 * 8 ? 65535 * 8 : (int)g0;
+  hipLaunchKernelGGL(energy_terms_kernel, dim3(g), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, f[0],
+                     f[1], f[2], f[3], f[4], ke, mphi);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TUPAN_OK : hip_fail(e, "energy terms launch");
 }
 
 int tupan_diagnostics_dev(size_t n, const double *const *f, int pn,

@@ -140,6 +140,26 @@ class System(object):
     def total_mass(self):
         return self.sum("mass")
 
+    def sum_np(self, name):
+        """Device sum of one field in numpy's order: the bits of the
+        reference drivers' ``ps.<name>.sum()``."""
+        out = torch.empty(1, dtype=torch.float64, device=self.device)
+        _lib.call_update("sum_pairwise", self.n, self.field(name).data_ptr(),
+                         out.data_ptr(), _stream(self))
+        return float(out.item())
+
+    def energies_np(self):
+        """(kinetic, potential) energy as the reference's ``ps.kinetic_energy``
+        and ``ps.potential_energy`` compute them (body.py:261-306, no PN
+        terms): per-particle terms summed in numpy's order, Phi in the
+        set's mode."""
+        self.set_phi()
+        _lib.call_update("energy_terms", self.n,
+                         self.ptrs(("mass", "vx", "vy", "vz", "phi")),
+                         self.field("_ke").data_ptr(),
+                         self.field("_mphi").data_ptr(), _stream(self))
+        return self.sum_np("_ke"), 0.5 * self.sum_np("_mphi")
+
     def diagnostics(self, with_phi=True):
         """The totals of the reference's report (body.py:79-322: total
         mass, kinetic, potential and virial energy, centre-of-mass position
@@ -509,10 +529,12 @@ class NREG(Base):
         self.is_initialized = False
 
     def initialize(self):
-        """nreg.py:114-126: E0 = KE + PE, W = U = S = -PE."""
-        d = self.ps.diagnostics()
-        self.E0 = d["kinetic_energy"] + d["potential_energy"]
-        self.W = self.U = self.S = -d["potential_energy"]
+        """nreg.py:114-126: E0 = KE + PE, W = U = S = -PE (sums in numpy's
+        order, as every sum of this driver: with the pair kernels in the
+        reference's arithmetic a run reproduces the reference's bits)."""
+        ke, pe = self.ps.energies_np()
+        self.E0 = ke + pe
+        self.W = self.U = self.S = -pe
         self.is_initialized = True
 
     def evolve_step(self, t_end):
@@ -527,7 +549,7 @@ class NREG(Base):
         ps._pair("nreg_x", dt)
         _lib.call_update("divide", ps.n, 3, ps.ptrs(("mrx", "mry", "mrz")),
                          ps.ptrs(("rx", "ry", "rz")), float(mtot), _stream(ps))
-        self.U = 0.5 * ps.sum("u")
+        self.U = 0.5 * ps.sum_np("u")
         ps.t_curr += dt
 
     def nreg_v(self, ps, dt):
@@ -536,7 +558,7 @@ class NREG(Base):
         ps._pair("nreg_v", dt)
         _lib.call_update("divide", ps.n, 3, ps.ptrs(("mvx", "mvy", "mvz")),
                          ps.ptrs(("vx", "vy", "vz")), float(mtot), _stream(ps))
-        kin = 0.25 * ps.sum("mk") / mtot
+        kin = 0.25 * ps.sum_np("mk") / mtot
         self.W = kin - self.E0
 
     def anreg_step(self, ps, h):
@@ -553,7 +575,7 @@ class NREG(Base):
 
     def do_step(self, ps, tau):
         """nreg.py:149-163."""
-        self._mtot = ps.total_mass()
+        self._mtot = ps.sum_np("mass")
         t0 = ps.t_curr
         if "anreg" in self.method:
             self.anreg_step(ps, tau / 2)
