@@ -56,7 +56,7 @@ def parse():
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--workload", default="accjerk",
                    choices=["accjerk", "sizes", "ensemble", "pn", "diag",
-                            "nreg"],
+                            "nreg", "adjoint"],
                    help="accjerk: the headline single-evaluation metric; "
                         "sizes: Acc, AccJerk, SnapCrackle at N = 16K, 64K, "
                         "256K in both modes with errors vs FP64; "
@@ -64,7 +64,8 @@ def parse():
                         "Hermite-4) against looping the single-system path; "
                         "pn: the compact post-Newtonian cluster with Sakura "
                         "close pairs; diag: the energy/momentum report at "
-                        "N = 65536; nreg: NREG steps at N = 1024")
+                        "N = 65536; nreg: NREG steps at N = 1024; adjoint: "
+                        "the differentiation workload at N = 16384")
     p.add_argument("--pmc-target", action="store_true",
                    help="one evaluation per mode plus a 256 MiB device copy "
                         "(the byte-counter calibration kernel), for "
@@ -509,6 +510,90 @@ def bench_nreg(args, torch):
     print(json.dumps(result))
 
 
+def bench_adjoint(args, torch):
+    """The differentiation workload: Plummer N=16384, seed 3, eps^2 = 1e-4,
+    target field from the masses perturbed by a seeded 5% Gaussian,
+    L = sum_i |a_i - a_target,i|^2; one gradient = forward Acc + the
+    adjoint pair sum (tupan_amd.autograd), FP32C and FP64; and the 64-step
+    leapfrog rollout (dt = 1/128, checkpoint every 8) with the loss on the
+    final positions.  The FP32C gradient's error is taken against the FP64
+    one (the FP64 one is pinned to central differences by the tests).
+    cpu_baseline: the adjoint pair sum restated in numpy (the reference has
+    no differentiation), one host thread, the full sum."""
+    from tupan_amd import autograd as ag
+    from tupan_amd import kernels
+    from tupan_amd.ics import make_plummer
+    n, seed, eps2 = 16384, 3, 1e-4
+    phi = lambda p: kernels.compute("phi", p, p, mode="fp64")[0]
+    ps = make_plummer(n, np.sqrt(eps2), seed=seed, phi_fn=phi)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    pos = torch.stack([dev(ps.rx), dev(ps.ry), dev(ps.rz)]).contiguous()
+    vel = torch.stack([dev(ps.vx), dev(ps.vy), dev(ps.vz)]).contiguous()
+    rng = np.random.RandomState(seed + 100)
+    mass, e2 = dev(ps.mass), dev(ps.eps2)
+    mt = dev(ps.mass * (1 + 0.05 * rng.standard_normal(n)))
+    at = ag.acc(pos, mt, e2, "fp64").detach()
+
+    def grad(mode):
+        p = pos.clone().requires_grad_(True)
+        m = mass.clone().requires_grad_(True)
+        ((ag.acc(p, m, e2, mode) - at)**2).sum().backward()
+        return p.grad, m.grad
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, out
+
+    reps = max(5, args.steps // 3)
+    t32, (gp32, gm32) = timed(lambda: grad("fp32c"), reps)
+    t64, (gp64, gm64) = timed(lambda: grad("fp64"), reps)
+    err_p = float((gp32 - gp64).abs().max() / gp64.abs().max())
+    err_m = float((gm32 - gm64).abs().max() / gm64.abs().max())
+
+    def rollout():
+        p = pos.clone().requires_grad_(True)
+        m = mass.clone().requires_grad_(True)
+        pf, _ = ag.leapfrog_rollout(p, vel, m, e2, 1.0 / 128, 64,
+                                    checkpoint_every=8, mode="fp32c")
+        (pf**2).sum().backward()
+        return p.grad
+    troll, _ = timed(rollout, 1)
+    from oracle import adjoint as oad
+    d = {f: np.ascontiguousarray(getattr(ps, f)) for f in
+         ("mass", "eps2", "rx", "ry", "rz")}
+    g = np.random.RandomState(0).standard_normal((3, n))
+    t = time.perf_counter()
+    oad.acc_adjoint(d, g)
+    cpu = time.perf_counter() - t
+    pairs = float(n) * n
+    result = {
+        "metric": "Acc-loss gradient (forward + adjoint), N=16384, FP32C",
+        "value": 1e3 * t32, "unit": "ms", "n_gpus": 1, "steps": reps,
+        "warmup": 1, "ms_per_step": 1e3 * t32, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "Plummer N=16384 seed 3, eps^2=1e-4, target "
+                               "masses 5% Gaussian (seed 103), "
+                               "L=sum|a-a_t|^2; 64-step leapfrog rollout "
+                               "dt=1/128, checkpoint 8"},
+        "pair_sums_Gpairs_s": 2 * pairs / t32 / 1e9,
+        "fp64_gradient_ms": 1e3 * t64,
+        "fp32c_vs_fp64_gradient_err": {"positions": err_p, "masses": err_m},
+        "rollout_fwd_bwd_ms": 1e3 * troll,
+        "cpu_baseline": {"value": 1e3 * cpu, "unit": "ms", "cores": 1,
+                         "kind": "port",
+                         "sample": "the adjoint pair sum alone (numpy "
+                                   "restatement, all %d x %d pairs), 1 "
+                                   "thread" % (n, n)},
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -535,6 +620,9 @@ def main():
         return
     if args.workload == "nreg":
         bench_nreg(args, torch)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args, torch)
         return
     n = args.n
     print("[bench] building Plummer N=%d workload" % n, file=sys.stderr, flush=True)
