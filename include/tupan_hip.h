@@ -344,6 +344,11 @@ int tupan_sum_pairwise_dev(size_t n, const double *x, double *out,
  * does, for totals summed in numpy's order. */
 int tupan_energy_terms_dev(size_t n, const double *const *f, double *ke,
                            double *mphi, void *stream);
+/* The twelve per-particle quantities whose totals tupan_diagnostics_dev
+ * returns (same f and pn), written to out[0..11] for totals summed in
+ * numpy's order (tupan_sum_pairwise_dev): the reference report's bits. */
+int tupan_report_terms_dev(size_t n, const double *const *f, int pn,
+                           double *const *out, void *stream);
 int tupan_diagnostics_dev(size_t n, const double *const *f, int pn,
                           double *out, void *stream);
 

@@ -151,6 +151,30 @@ def test_report_columns_match_checker(K):
             assert abs(got[k] - want[k]) <= 1e-12, k
 
 
+def test_report_bitwise_in_ieee_mode(K):
+    """In the reference-arithmetic mode the state stays bit-identical to the
+    checker's run on the reference's kernels, and the report sums each
+    per-particle quantity in numpy's order: every column of every report
+    (except wall-clock time) equals the checker's report exactly."""
+    from tupan_amd.diagnostics import Diagnostic
+    from tupan_amd.integrator import Integrator
+    d = _plummer(300, 15, 0.01)
+    s = _system(d)
+    integ = Integrator(1.0 / 64, 0.0, s, "hermite4", mode="fp64_ieee")
+    rep = Diagnostic(None, 0.0, s, report_freq=1)
+    rep.initialize(s)
+    host = dict(d)
+    host["t_curr"] = 0.0
+    ref = D.Diagnostic(K, host)
+    for k in range(4):
+        integ.evolve_step(1.0)
+        I.hermite_step(K, host, 1.0 / 64, 4)
+        got, want = rep.diagnostic_report(s), ref.report(host)
+        for c in ("ke", "pe", "te", "virial", "eerr", "geerr", "com_r",
+                  "com_v", "lmom", "amom"):
+            assert got[c] == want[c], (k, c, got[c], want[c])
+
+
 @pytest.mark.parametrize("method", ["nreg", "anreg"])
 def test_nreg_matches_checker(K, method):
     from tupan_amd.integrator import Integrator

@@ -113,6 +113,7 @@ def lib():
         L.tupan_sum_dev.argtypes = [_SZ, _P, _P, _P]
         L.tupan_sum_pairwise_dev.argtypes = [_SZ, _P, _P, _P]
         L.tupan_energy_terms_dev.argtypes = [_SZ, _PP, _P, _P, _P]
+        L.tupan_report_terms_dev.argtypes = [_SZ, _PP, ctypes.c_int, _PP, _P]
         L.tupan_diagnostics_dev.argtypes = [_SZ, _PP, ctypes.c_int, _P, _P]
         for name in ("nreg_x", "nreg_v"):
             getattr(L, "tupan_%s_dev" % name).argtypes = [

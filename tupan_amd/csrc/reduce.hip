@@ -14,6 +14,9 @@
 //                          with 8 interleaved accumulators
 //   tupan_energy_terms_dev the per-particle 0.5 m v^2 and m phi of body.py
 //                          (:261-275, :294-300), for sums in numpy's order
+//   tupan_report_terms_dev the report's twelve per-particle quantities
+//                          (those of tupan_diagnostics_dev), stored for sums
+//                          in numpy's order
 //
 // Two launches: B <= 256 blocks accumulate grid-strided per-thread sums and
 // reduce them through LDS in a fixed tree; one block adds the B partials
@@ -220,6 +223,22 @@ __global__ void np_final_kernel(const double *part, int nchunks, double *out) {
   out[0] = r;
 }
 
+// The report's per-particle quantities (DiagFn's, stored instead of
+// summed) for totals in numpy's order.
+template <bool PN>
+__global__ __launch_bounds__(256) void report_terms_kernel(int n, Fields p,
+                                                          Fields out) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n;
+       i += gridDim.x * 256) {
+    double a[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) a[k] = 0.0;
+    DiagFn<PN>::add(p, i, a);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) const_cast<double *>(out.f[k])[i] = a[k];
+  }
+}
+
 __global__ __launch_bounds__(256) void energy_terms_kernel(
     int n, const double *m, const double *vx, const double *vy,
     const double *vz, const double *phi, double *ke, double *mphi) {
@@ -282,6 +301,35 @@ int tupan_energy_terms_dev(size_t n, const double *const *f, double *ke,
                      f[1], f[2], f[3], f[4], ke, mphi);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? TUPAN_OK : hip_fail(e, "energy terms launch");
+}
+
+int tupan_report_terms_dev(size_t n, const double *const *f, int pn,
+                           double *const *out, void *stream) {
+  if (n > (size_t)INT_MAX)
+    return set_error(TUPAN_ERR_ARG, "n too large (%zu)", n);
+  if (!f || !out) return set_error(TUPAN_ERR_ARG, "NULL field array");
+  const int nf = pn ? 18 : 8;
+  Fields p{}, o{};
+  for (int k = 0; k < nf; ++k) {
+    if (n && !f[k]) return set_error(TUPAN_ERR_ARG, "NULL field %d", k);
+    p.f[k] = f[k];
+  }
+  for (int k = 0; k < 12; ++k) {
+    if (n && !out[k]) return set_error(TUPAN_ERR_ARG, "NULL output %d", k);
+    o.f[k] = out[k];
+  }
+  if (n == 0) return TUPAN_OK;
+  const size_t g0 = (n + 255) / 256;
+  const int g = g0 > 65535 * 8 ? 65535 * 8 : (int)g0;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (pn)
+    hipLaunchKernelGGL(report_terms_kernel<true>, dim3(g), dim3(256), 0, st,
+                       (int)n, p, o);
+  else
+    hipLaunchKernelGGL(report_terms_kernel<false>, dim3(g), dim3(256), 0, st,
+                       (int)n, p, o);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TUPAN_OK : hip_fail(e, "report terms launch");
 }
 
 int tupan_diagnostics_dev(size_t n, const double *const *f, int pn,

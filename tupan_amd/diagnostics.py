@@ -3,9 +3,11 @@
 ``Diagnostic(fname, time, ps, report_freq, pn_order)`` takes a device
 ``System`` and prints the reference's columns (time, dtime, ke, pe, te,
 virial, eerr, geerr, com_r, com_v, lmom, amom, wct) with the reference's
-format.  Each report costs one FP64 Phi evaluation and one fused reduction
-over the particles (``System.diagnostics``), where the reference evaluates
-Phi and then about a dozen numpy reductions on the host.
+format.  Each report costs one Phi evaluation and the report's twelve
+totals summed in numpy's order on the device (``System.diagnostics``), so
+that a run in the reference-arithmetic mode prints the reference's numbers
+bit for bit; the reference evaluates Phi and then about a dozen numpy
+reductions on the host.
 """
 from __future__ import print_function
 
@@ -55,7 +57,7 @@ class Diagnostic(object):
         self.is_initialized = False
 
     def initialize(self, ps):
-        d = ps.diagnostics()
+        d = ps.diagnostics(np_order=True)
         self.ke0 = d["kinetic_energy"]
         self.pe0 = d["potential_energy"]
         self.te0 = self.ke0 + self.pe0
@@ -89,7 +91,7 @@ class Diagnostic(object):
 
     def print_diagnostic(self, time, dtime, ps):
         self.time = time
-        d = ps.diagnostics()
+        d = ps.diagnostics(np_order=True)
         ke = d["kinetic_energy"]
         pe = d["potential_energy"]
         te = ke + pe

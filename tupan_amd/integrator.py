@@ -160,23 +160,33 @@ class System(object):
                          self.field("_mphi").data_ptr(), _stream(self))
         return self.sum_np("_ke"), 0.5 * self.sum_np("_mphi")
 
-    def diagnostics(self, with_phi=True):
+    def diagnostics(self, with_phi=True, np_order=False):
         """The totals of the reference's report (body.py:79-322: total
         mass, kinetic, potential and virial energy, centre-of-mass position
         and velocity, linear and angular momentum, PN accumulators included
-        when enabled) from one fused device pass; ``with_phi`` first
-        evaluates Phi in FP64 (``ps.set_phi(ps)``, body.py:298)."""
+        when enabled); ``with_phi`` first evaluates Phi (``ps.set_phi(ps)``,
+        body.py:298; FP64, or the set's mode when that is fp64_ieee).  By
+        default one fused device pass; ``np_order`` sums each per-particle
+        quantity in numpy's order instead, so that with bit-identical
+        inputs the totals are the reference's bits."""
         if with_phi:
-            self.set_phi(mode="fp64")
+            self.set_phi(mode="fp64_ieee" if self.mode == "fp64_ieee"
+                         else "fp64")
         names = ("mass", "rx", "ry", "rz", "vx", "vy", "vz", "phi")
         pn = bool(self.include_pn_corrections)
         if pn:
             names += ("pn_ke", "pn_mrx", "pn_mry", "pn_mrz", "pn_mvx",
                       "pn_mvy", "pn_mvz", "pn_amx", "pn_amy", "pn_amz")
-        out = torch.empty(12, dtype=torch.float64, device=self.device)
-        _lib.call_update("diagnostics", self.n, self.ptrs(names), int(pn),
-                         out.data_ptr(), _stream(self))
-        s = out.cpu().numpy()
+        if np_order:
+            terms = ["_q%d" % k for k in range(12)]
+            _lib.call_update("report_terms", self.n, self.ptrs(names),
+                             int(pn), self.ptrs(terms), _stream(self))
+            s = np.array([self.sum_np(t) for t in terms])
+        else:
+            out = torch.empty(12, dtype=torch.float64, device=self.device)
+            _lib.call_update("diagnostics", self.n, self.ptrs(names),
+                             int(pn), out.data_ptr(), _stream(self))
+            s = out.cpu().numpy()
         mtot = float(s[0])
         ke = float(s[1])
         pe = 0.5 * float(s[2])
