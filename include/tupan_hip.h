@@ -100,6 +100,15 @@ int tupan_set_near_field(double c);
  * reported as not converged. */
 int tupan_set_sakura_budget(int solves);
 int tupan_take_kernel_times(double *total_ms, int *count);
+/* The asynchronous entry points keep their scratch (packed j records,
+ * partial sums) in per-(device, stream) buffers that grow to the largest
+ * call and are reused, so steady-state calls allocate nothing and can be
+ * captured into a HIP graph once the same call has run outside the
+ * capture.  A graph shares these buffers with later calls on its capture
+ * stream: do not replay it concurrently with them.  tupan_release_scratch
+ * synchronises the devices and frees every buffer (graphs captured before
+ * must not be replayed afterwards). */
+int tupan_release_scratch(void);
 
 /* ---- Phi: replaces phi_kernel, tupan/lib/src/libtupan.h:2-15 ---------
  * ip, jp: F5.  op: phi. */

@@ -149,20 +149,20 @@ int run_ieee_engine(size_t ni, const double *const *ip, size_t nj,
   if (n_j > 0) {
     const size_t bytes =
         ((sizeof(typename Pol::JRec) * (size_t)n_j + 255) / 256) * 256;
-    const int rc = device_alloc((void **)&jr, bytes, st);
+    const int rc = scratch_alloc(kSlotIeee, (void **)&jr, bytes, st);
     if (rc) return rc;
     hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
                        dim3(256), 0, st, js, n_j, jr, (const double *)nullptr);
   }
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  const bool timed = timing_begin(&ev0, &ev1);
+  const bool timed = timing_begin(&ev0, &ev1, st);
   if (timed) (void)hipEventRecord(ev0, st);
   hipLaunchKernelGGL((ieee_engine<Pol, U>), dim3((n_i + 255) / 256),
                      dim3(256), 0, st, is, n_i,
                      (const typename Pol::JRec *)jr, n_j, out, prm);
   if (timed) (void)hipEventRecord(ev1, st);
   hipError_t e = hipGetLastError();
-  device_free(jr, st);
+  scratch_free(kSlotIeee, jr, st);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return TUPAN_OK;
 }

@@ -22,9 +22,16 @@ LaunchShape choose_shape(int kernel_id, int mode, size_t ni, size_t nj,
                          int pref_ipl, int max_ipl);
 int device_alloc(void **p, size_t bytes, hipStream_t st);
 void device_free(void *p, hipStream_t st);
+// Reusable per-(device, stream, slot) device buffers for the asynchronous
+// entry points (no allocation in steady state; safe to capture).  Each
+// buffer in simultaneous use within one call needs its own slot.
+enum ScratchSlot { kSlotRecords = 0, kSlotPartials, kSlotIeee, kSlotReduce,
+                   kSlotPairwise };
+int scratch_alloc(int slot, void **p, size_t bytes, hipStream_t st);
+void scratch_free(int slot, void *p, hipStream_t st);
 // Optional live timing of the pair-engine launches (tupan_set_timing):
 // returns the event pair to record around the launch, or nulls when off.
-bool timing_begin(hipEvent_t *e0, hipEvent_t *e1);
+bool timing_begin(hipEvent_t *e0, hipEvent_t *e1, hipStream_t st);
 
 using DevFn = std::function<int(const double *const *, const double *const *,
                                 double *const *, hipStream_t)>;
@@ -113,18 +120,18 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
       Pol::kTwoLevel ? sizeof(double) * (2 + 4 * ntiles + 2 * kCtxBlocks)
                      : 0;
   if (n_j > 0) {
-    rc = device_alloc((void **)&jr, rec_bytes + ctx_bytes, st);
+    rc = scratch_alloc(kSlotRecords, (void **)&jr, rec_bytes + ctx_bytes, st);
     if (rc) return rc;
     if (Pol::kTwoLevel)
       ctx = reinterpret_cast<double *>(reinterpret_cast<char *>(jr) +
                                        rec_bytes);
   }
   if (SB > 1) {
-    rc = device_alloc((void **)&part,
-                      sizeof(double) * (size_t)SB * Pol::kOut * (size_t)n_i,
-                      st);
+    rc = scratch_alloc(kSlotPartials, (void **)&part,
+                       sizeof(double) * (size_t)SB * Pol::kOut * (size_t)n_i,
+                       st);
     if (rc) {
-      device_free(jr, st);
+      scratch_free(kSlotRecords, jr, st);
       return rc;
     }
   }
@@ -146,7 +153,7 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   dim3 grid((unsigned)waves, (unsigned)SB);
   const double *aux = ctx ? ctx + 2 : nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  const bool timed = timing_begin(&ev0, &ev1);
+  const bool timed = timing_begin(&ev0, &ev1, st);
   if (timed) (void)hipEventRecord(ev0, st);
   switch (P) {
     case 4:
@@ -165,8 +172,8 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     hipLaunchKernelGGL((finish_kernel<Pol>), dim3((n_i + 255) / 256),
                        dim3(256), 0, st, part, SB, n_i, out, prm);
   hipError_t e = hipGetLastError();
-  device_free(part, st);
-  device_free(jr, st);
+  scratch_free(kSlotPartials, part, st);
+  scratch_free(kSlotRecords, jr, st);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return TUPAN_OK;
 }

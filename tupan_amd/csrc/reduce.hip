@@ -141,14 +141,15 @@ int run_reduce(const Fields &p, size_t n, double *out, hipStream_t st) {
   const size_t nb0 = (n + kRThreads - 1) / kRThreads;
   const int nb = nb0 < 1 ? 1 : (nb0 > kMaxPartials ? kMaxPartials : (int)nb0);
   double *part = nullptr;
-  int rc = device_alloc((void **)&part, sizeof(double) * K * nb, st);
+  int rc = scratch_alloc(kSlotReduce, (void **)&part, sizeof(double) * K * nb,
+                         st);
   if (rc) return rc;
   hipLaunchKernelGGL((reduce_partial<Fn>), dim3(nb), dim3(kRThreads), 0, st,
                      p, (int)n, part);
   hipLaunchKernelGGL((reduce_final<K>), dim3(1), dim3(kRThreads), 0, st,
                      (const double *)part, nb, out);
   hipError_t e = hipGetLastError();
-  device_free(part, st);
+  scratch_free(kSlotReduce, part, st);
   if (e != hipSuccess) return hip_fail(e, "reduction launch");
   return TUPAN_OK;
 }
@@ -273,7 +274,8 @@ int tupan_sum_pairwise_dev(size_t n, const double *x, double *out,
   const int nchunks = (int)((n + kNpChunk - 1) / kNpChunk);
   double *part = nullptr;
   if (nchunks > 0) {
-    int rc = device_alloc((void **)&part, sizeof(double) * nchunks, st);
+    int rc = scratch_alloc(kSlotPairwise, (void **)&part,
+                           sizeof(double) * nchunks, st);
     if (rc) return rc;
     hipLaunchKernelGGL(np_chunk_kernel, dim3(nchunks), dim3(64), 0, st, x,
                        (int)n, part);
@@ -281,7 +283,7 @@ int tupan_sum_pairwise_dev(size_t n, const double *x, double *out,
   hipLaunchKernelGGL(np_final_kernel, dim3(1), dim3(1), 0, st,
                      (const double *)part, nchunks, out);
   hipError_t e = hipGetLastError();
-  device_free(part, st);
+  scratch_free(kSlotPairwise, part, st);
   if (e != hipSuccess) return hip_fail(e, "pairwise sum launch");
   return TUPAN_OK;
 }

@@ -3,7 +3,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "launch.hpp"
@@ -92,13 +94,17 @@ int device_alloc(void **p, size_t bytes, hipStream_t st) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-  if (dev >= 0 && dev < 64 && !pool_set[dev]) {
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-      uint64_t thr = UINT64_MAX;
-      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (dev >= 0 && dev < 64 && !pool_set[dev]) {
+      hipMemPool_t pool;
+      if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold,
+                                     &thr);
+      }
+      pool_set[dev] = true;
     }
-    pool_set[dev] = true;
   }
   if (bytes == 0) bytes = 8;
   e = hipMallocAsync(p, bytes, st);
@@ -110,11 +116,89 @@ void device_free(void *p, hipStream_t st) {
   if (p) (void)hipFreeAsync(p, st);
 }
 
+// ------------------------------------------------------ scratch cache
+// Per (device, stream, slot) buffers reused from call to call: calls on one
+// stream run in order, so a buffer serves the next call as soon as the
+// previous call's kernels are enqueued, and steady-state calls allocate
+// nothing (which also lets a caller capture them into a HIP graph).
+// Buffers only grow, and a replaced buffer is retired, not freed, until
+// tupan_release_scratch: kernels captured into a graph keep valid pointers.
+// A buffer in use by another host thread (same stream and slot) is not
+// shared: that call allocates its own and frees it stream-ordered.
+struct ScratchEntry {
+  void *p = nullptr;
+  size_t bytes = 0;
+  bool busy = false;
+};
+static std::map<std::tuple<int, hipStream_t, int>, ScratchEntry> g_scratch;
+static std::vector<std::pair<int, void *>> g_retired;  // (device, buffer)
+
+static bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) return false;
+  return cs != hipStreamCaptureStatusNone;
+}
+
+int scratch_alloc(int slot, void **p, size_t bytes, hipStream_t st) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (bytes == 0) bytes = 8;
+  const bool cap = capturing(st);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ScratchEntry &s = g_scratch[std::make_tuple(dev, st, slot)];
+    if (!s.busy && s.bytes >= bytes) {
+      s.busy = true;
+      *p = s.p;
+      return TUPAN_OK;
+    }
+    if (!s.busy) {
+      if (cap)
+        return set_error(TUPAN_ERR_ARG,
+                         "scratch buffer too small during stream capture: "
+                         "make the same call once before capturing");
+      size_t nb = bytes > s.bytes + s.bytes / 2 ? bytes : s.bytes + s.bytes / 2;
+      nb = (nb + 255) / 256 * 256;
+      void *q = nullptr;
+      e = hipMallocAsync(&q, nb, st);
+      if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+      if (s.p) g_retired.emplace_back(dev, s.p);
+      s.p = q;
+      s.bytes = nb;
+      s.busy = true;
+      *p = q;
+      return TUPAN_OK;
+    }
+  }
+  if (cap)
+    return set_error(TUPAN_ERR_ARG,
+                     "scratch buffer busy during stream capture");
+  return device_alloc(p, bytes, st);
+}
+
+void scratch_free(int slot, void *p, hipStream_t st) {
+  if (!p) return;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_scratch.find(std::make_tuple(dev, st, slot));
+    if (it != g_scratch.end() && it->second.p == p) {
+      it->second.busy = false;
+      return;
+    }
+  }
+  device_free(p, st);
+}
+
 // ------------------------------------------------------ live timing
 static bool g_timing = false;
 static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
 
-bool timing_begin(hipEvent_t *e0, hipEvent_t *e1) {
+// Launches being captured into a graph are not timed: their events would
+// be recorded only when the graph replays.
+bool timing_begin(hipEvent_t *e0, hipEvent_t *e1, hipStream_t st) {
+  if (capturing(st)) return false;
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_timing) return false;
   if (hipEventCreate(e0) != hipSuccess) return false;
@@ -226,6 +310,34 @@ int tupan_set_sakura_budget(int solves) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_sakura_budget = solves;
   return TUPAN_OK;
+}
+
+int tupan_release_scratch(void) {
+  std::vector<std::pair<int, void *>> all;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto &kv : g_scratch) {
+      if (kv.second.busy)
+        return set_error(TUPAN_ERR_ARG, "a scratch buffer is in use");
+    }
+    for (auto &kv : g_scratch)
+      if (kv.second.p) all.emplace_back(std::get<0>(kv.first), kv.second.p);
+    all.insert(all.end(), g_retired.begin(), g_retired.end());
+    g_scratch.clear();
+    g_retired.clear();
+  }
+  int cur = 0;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  int rc = TUPAN_OK;
+  for (auto &dp : all) {
+    if (hipSetDevice(dp.first) != hipSuccess) continue;
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipFree(dp.second);
+    if (e != hipSuccess && rc == TUPAN_OK) rc = hip_fail(e, "hipFree");
+  }
+  (void)hipSetDevice(cur);
+  return rc;
 }
 
 int tupan_set_timing(int on) {

@@ -228,10 +228,22 @@ class Base(object):
 
     PROVIDED_METHODS = []
 
+    # methods whose step makes no host round trip and can be replayed as a
+    # HIP graph (fixed step lengths)
+    GRAPH_METHODS = ()
+
     def __init__(self, eta, time, ps, method, pn_order=0, clight=None,
-                 mode="fp64"):
+                 mode="fp64", graph=False):
         if method not in self.PROVIDED_METHODS:
             raise ValueError("Unexpected method: {0}".format(method))
+        if graph and method not in self.GRAPH_METHODS:
+            raise ValueError("method {0} takes host decisions inside a step "
+                             "and cannot be replayed as a graph"
+                             .format(method))
+        self.graph = bool(graph)
+        self._graphs = {}
+        self._seen = set()
+        self._gstream = None
         self.eta = eta
         self.method = method
         ps.t_curr = time
@@ -273,7 +285,40 @@ class Base(object):
 
     def evolve_step(self, t_end):
         tau = self.get_base_tstep(t_end)
-        self.ps = self.do_step(self.ps, tau)
+        if self.graph:
+            self.ps = self._graph_step(self.ps, tau)
+        else:
+            self.ps = self.do_step(self.ps, tau)
+
+    def _graph_step(self, ps, tau):
+        """One step of length ``tau`` as a HIP graph replay.  The first step
+        of a given length runs eagerly on the capture stream, so that every
+        field and library scratch buffer it uses exists there; the second
+        captures the step (recorded, not executed) and replays it, and later
+        ones replay.  The host clock advances outside the graph.  Scalars
+        (tau, eta, clight) are recorded into the graph: a step length is
+        captured once and replayed as recorded."""
+        g = self._graphs.get(tau)
+        if g is None:
+            cur = torch.cuda.current_stream(ps.device)
+            if self._gstream is None:
+                self._gstream = torch.cuda.Stream(device=ps.device)
+            self._gstream.wait_stream(cur)
+            if tau not in self._seen:
+                with torch.cuda.stream(self._gstream):
+                    ps = self.do_step(ps, tau)
+                cur.wait_stream(self._gstream)
+                self._seen.add(tau)
+                return ps
+            t0 = ps.t_curr
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self._gstream):
+                self.do_step(ps, tau)
+            ps.t_curr = t0
+            self._graphs[tau] = g
+        g.replay()
+        ps.t_curr += tau
+        return ps
 
     def evolve(self, t_end, callback=None):
         """Step until t_end (tupan/simulation.py:190-193); ``callback(ps)``
@@ -302,6 +347,7 @@ class Hermite(Base):
 
     PROVIDED_METHODS = ["hermite2", "ahermite2", "hermite4", "ahermite4",
                         "hermite6", "ahermite6", "hermite8", "ahermite8"]
+    GRAPH_METHODS = ("hermite2", "hermite4", "hermite6", "hermite8")
 
     def __init__(self, eta, time, ps, method, **kw):
         super(Hermite, self).__init__(eta, time, ps, method, **kw)
@@ -353,6 +399,7 @@ class SIA(Base):
 
     PROVIDED_METHODS = ["sia21s.dkd", "sia21a.dkd", "sia21s.kdk",
                         "sia21a.kdk"]
+    GRAPH_METHODS = ("sia21s.dkd", "sia21s.kdk")
 
     def __init__(self, eta, time, ps, method, **kw):
         super(SIA, self).__init__(eta, time, ps, method, **kw)
@@ -418,6 +465,7 @@ class Sakura(Base):
     """Sakura pairwise Kepler splitting (sakura.py:22-162)."""
 
     PROVIDED_METHODS = ["sakura", "asakura"]
+    GRAPH_METHODS = ("sakura",)
 
     def sakura_step(self, ps, tau):
         r = ps.ptrs(("rx", "ry", "rz"))
@@ -467,6 +515,7 @@ class SakuraPN(SIA):
     Shared fixed steps tau = min(t_end - t, eta)."""
 
     PROVIDED_METHODS = ["sakura_pn"]
+    GRAPH_METHODS = ("sakura_pn",)
 
     def __init__(self, eta, time, ps, method, rcut=1e-3, **kw):
         super(SIA, self).__init__(eta, time, ps, method, **kw)
