@@ -96,6 +96,22 @@ __device__ __forceinline__ uint32_t near_bit(uint64_t far) {
   return b;
 }
 
+// Policies whose pairs are mostly no-ops (kSparse: Sakura restricted to
+// the pairs closer than a cut-off) provide near(q, r, prm), true for the
+// pairs pair() would change, and sparse(prm), true when that test is worth
+// a pass of its own.  The engine then tests a tile of kTile j's first and
+// calls pair() only for the j's some lane needs, in j order: each lane
+// sees the same sequence of contributing pairs, so the sums are the same
+// bits as those of the dense loop.
+template <class Pol, class = void>
+struct SparsePairs {
+  static constexpr bool value = false;
+};
+template <class Pol>
+struct SparsePairs<Pol, decltype((void)Pol::kSparse)> {
+  static constexpr bool value = Pol::kSparse;
+};
+
 // ------------------------------------------------------------- engine
 // FP32C policies (kTwoLevel) also receive the caller's FP64 arrays and the
 // j index: a pair they classify as "near" is re-evaluated from those in FP64.
@@ -178,14 +194,53 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
         }
       }
     }
-  } else if (jb < je) {
-    typename Pol::JRec r = jr[jb];
-#pragma unroll 2
-    for (int j = jb; j < je; ++j) {
-      const typename Pol::JRec rn = jr[min(j + 1, je - 1)];
+  } else {
+    bool dense = true;
+    if constexpr (SparsePairs<Pol>::value) {
+      if (Pol::sparse(prm)) {
+        dense = false;
+        for (int j0 = jb; j0 < je; j0 += kTile) {
+          const int nk = min(kTile, je - j0);
+          uint32_t need = 0u;  // wave-uniform: bit k = some lane needs j0 + k
+          if (nk == kTile) {
+#pragma unroll 8
+            for (int k = 0; k < kTile; ++k) {
+              const typename Pol::JRec r = jr[j0 + k];
+              bool nb = false;
 #pragma unroll
-      for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
-      r = rn;
+              for (int p = 0; p < P; ++p)
+                nb = nb || Pol::near(ist[p], r, prm);
+              need |= (__any(nb) ? 1u : 0u) << k;
+            }
+          } else {
+            for (int k = 0; k < nk; ++k) {
+              const typename Pol::JRec r = jr[j0 + k];
+              bool nb = false;
+#pragma unroll
+              for (int p = 0; p < P; ++p)
+                nb = nb || Pol::near(ist[p], r, prm);
+              need |= (__any(nb) ? 1u : 0u) << k;
+            }
+          }
+          while (need) {
+            const int k = __builtin_ctz(need);
+            need &= need - 1u;
+            const typename Pol::JRec r = jr[j0 + k];
+#pragma unroll
+            for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
+          }
+        }
+      }
+    }
+    if (dense && jb < je) {
+      typename Pol::JRec r = jr[jb];
+#pragma unroll 2
+      for (int j = jb; j < je; ++j) {
+        const typename Pol::JRec rn = jr[min(j + 1, je - 1)];
+#pragma unroll
+        for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
+        r = rn;
+      }
     }
   }
 

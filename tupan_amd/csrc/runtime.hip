@@ -73,7 +73,11 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
       o = g_override[kid][mode];
   }
   const long target = 64L * simd_count();
-  const long max_split = nj >= 256 ? (long)(nj / 128) : 1;
+  // j chunks of at least 128 records, or of 32 (one tile) while that keeps
+  // the split count at 64 or below: small systems need the extra waves.
+  long max_split = nj >= 256 ? (long)(nj / 128) : 1;
+  const long fine = std::min<long>((long)(nj / 32), 64);
+  if (fine > max_split) max_split = fine;
   int P = o.ipl > 0 ? o.ipl : pref_ipl;
   if (P > max_ipl) P = max_ipl;
   if (P != 1 && P != 2 && P != 4) P = 1;

@@ -36,13 +36,33 @@ struct SakuraF64 : SumOut<6> {
   }
   __device__ static void zero(A64 &a) { zero_acc(a); }
   __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
+  // Cut-off runs (finite rc2) test every pair before evolving any.
+  static constexpr bool kSparse = true;
+  __device__ static bool sparse(const Params &prm) { return prm.rc2 < INFINITY; }
+  // The pairs pair() evolves: r^2 < rc2, except the zero state (r = v = 0:
+  // a particle paired with itself, or an exact duplicate) of a pair whose
+  // weight m_j / m is finite, with a finite dt.  pair_evolve leaves the
+  // zero state at zero, so such a pair adds exactly +-0 to accumulators
+  // that start at +0, and skipping it changes no bit.  (A massless pair
+  // adds NaN, as in the reference, and is kept.)
+  __device__ static bool zero_state(double r2, double dvx, double dvy,
+                                    double dvz, double m, double dt) {
+    return !(r2 > 0) && dvx == 0 && dvy == 0 && dvz == 0 && m > 0 &&
+           m < INFINITY && fabs(dt) < INFINITY;
+  }
+  __device__ static bool near(const I &q, const JRec &r, const Params &prm) {
+    double dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    return r2 < prm.rc2 && !zero_state(r2, q.vx - r.vx, q.vy - r.vy,
+                                       q.vz - r.vz, q.m + r.m, prm.dt);
+  }
   __device__ static void pair(const I &q, const JRec &r, A64 &a,
                               const Params &prm) {
     double st[6] = {q.x - r.x,   q.y - r.y,   q.z - r.z,
                     q.vx - r.vx, q.vy - r.vy, q.vz - r.vz};
-    if (prm.rc2 < INFINITY &&
-        !(st[0] * st[0] + st[1] * st[1] + st[2] * st[2] < prm.rc2))
-      return;
+    const double r2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
+    if (prm.rc2 < INFINITY && !(r2 < prm.rc2)) return;
+    if (zero_state(r2, st[3], st[4], st[5], q.m + r.m, prm.dt)) return;
     const double r0[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
     double m = q.m + r.m;
     double e2 = q.e2 + r.e2;
