@@ -319,6 +319,17 @@ int tupan_kick_pn_half_dev(size_t n, double *const *v, const double *const *a,
  * pn_am -= (r x m pna) tau, w = 2 pna tau - w. */
 int tupan_pn_flux_dev(size_t n, const double *const *in, double *const *acc,
                       double tau, void *stream);
+/* tupan_pn_flux_dev followed by the second tupan_kick_pn_half_dev, fused
+ * (no reference counterpart; the same bits as the two calls):
+ * in = {m, rx,ry,rz, vx,vy,vz, pnax,pnay,pnaz} with v updated in place,
+ * acc as tupan_pn_flux_dev, a = {ax, ay, az} (the Newtonian part). */
+int tupan_pn_flux_kick_dev(size_t n, double *const *in, double *const *acc,
+                           const double *const *a, double tau, void *stream);
+/* end of a shared step (integrator/__init__.py and the drivers' clock
+ * updates): tstep[i] = tau, time[i] += tau, nstep[i] += 1 (no reference
+ * counterpart as one call). */
+int tupan_advance_clock_dev(size_t n, double *tstep, double *time,
+                            long long *nstep, double tau, void *stream);
 /* sakura_step's application of the pair deltas (sakura.py:31-36, 39-44):
  * x = {rx..vz} += d = {drx..dvz}. */
 int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,

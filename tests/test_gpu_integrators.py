@@ -133,6 +133,16 @@ def test_sia_and_sakura_update_kernels_bitwise(torch):
     for k, c in enumerate("xyz"):
         ps["pna" + c] = pna[k]
     check(ps, "kick_pn")
+    # kick_pn again, with the flux and the second half kick fused
+    oi.kick(_Given(pna), ps, tau, pn=True)
+    _lib.call_update("kick_pn_half", n, P(V), P(A), P(W), tau, 0)
+    _lib.call_update("pn_flux_kick", n,
+                     P(("mass",) + R + V + ("pnax", "pnay", "pnaz")),
+                     P(W + ("pn_ke", "pn_mvx", "pn_mvy", "pn_mvz",
+                            "pn_amx", "pn_amy", "pn_amz")), P(A), tau, 0)
+    for k, c in enumerate("xyz"):
+        ps["pna" + c] = pna[k]
+    check(ps, "kick_pn fused")
     # Sakura: half drift and delta application
     for dd in "xyz":
         ps["r" + dd] += ps["v" + dd] * tau / 2
@@ -145,6 +155,18 @@ def test_sia_and_sakura_update_kernels_bitwise(torch):
     _lib.call_update("apply_delta", n, P(R + V),
                      P(("drx", "dry", "drz", "dvx", "dvy", "dvz")), 0)
     check(ps, "apply_delta")
+    # end-of-step clock
+    tstep = _dev(torch, rng.normal(0, 1, n))
+    time = _dev(torch, rng.normal(0, 1, n))
+    nstep = torch.as_tensor(rng.randint(0, 1 << 40, n), device="cuda")
+    t_ref = time.cpu().numpy() + tau
+    n_ref = nstep.cpu().numpy() + 1
+    _lib.call_update("advance_clock", n, tstep.data_ptr(), time.data_ptr(),
+                     nstep.data_ptr(), tau, 0)
+    torch.cuda.synchronize()
+    assert np.all(tstep.cpu().numpy() == tau)
+    assert np.array_equal(time.cpu().numpy(), t_ref)
+    assert np.array_equal(nstep.cpu().numpy(), n_ref)
 
 
 def _cases():

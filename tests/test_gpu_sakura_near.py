@@ -1,0 +1,116 @@
+"""Sakura restricted to close pairs (sakura_near) on the device.
+
+* against the checker's sakura_near (the reference's C pair kernel on
+  every pair with r < rcut, self pairs included) on clusters with planted
+  close pairs, exact duplicates (the zero state the kernel skips, which the
+  reference evaluates to exactly zero) and sizes that are not multiples of
+  the 32-record tile or the 64-lane wave: within 1e-11 of each output's
+  scale (host and device run different sin/cos/sinh/cosh);
+* with a cut-off above every separation, sakura_near evaluates every pair
+  through the near mask and equals the full Sakura kernel (which runs the
+  dense loop) bit for bit: the mask selects exactly the contributing pairs
+  and keeps their order;
+* a cut-off of zero selects nothing: zero outputs, clean status.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+F8 = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
+OUT = ("drx", "dry", "drz", "dvx", "dvy", "dvz")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no HIP device")
+    return t
+
+
+@pytest.fixture(scope="module")
+def K():
+    from oracle import Oracle
+    from oracle import integrators as oi
+    return oi.Kernels(Oracle("float64", "port"))
+
+
+def _cluster(n, seed, close=6, dup=2):
+    rng = np.random.RandomState(seed)
+    d = {"mass": rng.uniform(0.5, 1.5, n) / n,
+         "eps2": np.zeros(n)}
+    for c in "xyz":
+        d["r" + c] = rng.standard_normal(n)
+        d["v" + c] = 0.3 * rng.standard_normal(n)
+    # close pairs: j placed at distance ~1e-3 of i
+    for k in range(close):
+        i, j = 2 * k, 2 * k + 1
+        off = rng.standard_normal(3)
+        off *= rng.uniform(2e-4, 8e-4) / np.linalg.norm(off)
+        for q, c in enumerate("xyz"):
+            d["r" + c][j] = d["r" + c][i] + off[q]
+            d["v" + c][j] = d["v" + c][i] + 0.05 * rng.standard_normal()
+    # exact duplicates of particle 20 + k
+    for k in range(dup):
+        for f in F8:
+            d[f][n - 1 - k] = d[f][20 + k]
+    return d
+
+
+def _dev(torch, d):
+    return {f: torch.as_tensor(np.ascontiguousarray(d[f]), device="cuda")
+            for f in F8}
+
+
+def _near(torch, t, dt, flag, rcut):
+    from tupan_amd import device
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    out = device.compute("sakura_near", t, t, dt, flag, rcut, status=st)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in out], int(st.item())
+
+
+@pytest.mark.parametrize("n,flag", [(77, 1), (300, -1), (1000, 1)])
+def test_sakura_near_matches_checker(torch, K, n, flag):
+    from oracle import integrators as oi
+    d = _cluster(n, n, close=6, dup=2)
+    rcut, dt = 1e-3, 1.0 / 512
+    got, status = _near(torch, _dev(torch, d), dt, flag, rcut)
+    assert status == 0
+    ref = {f: d[f].copy() for f in F8}
+    oi.sakura_near(K, ref, dt, flag, rcut)
+    touched = 0
+    for k, f in enumerate(OUT):
+        scale = np.abs(ref[f]).max()
+        assert scale > 0, f
+        assert np.abs(got[k] - ref[f]).max() <= 1e-11 * scale, f
+        touched = max(touched, int((ref[f] != 0).sum()))
+    assert touched >= 12  # the planted pairs moved, both members
+    for k in range(2):    # duplicates: zero delta from their twin
+        i = 20 + k
+        assert all(got[q][n - 1 - k] == got[q][i] for q in range(6))
+
+
+@pytest.mark.parametrize("n", [64, 333, 2048])
+def test_sakura_near_all_pairs_equals_full_sakura(torch, n):
+    from tupan_amd import device
+    d = _cluster(n, 7 + n, close=4, dup=2)
+    t = _dev(torch, d)
+    dt = 1.0 / 1024
+    got, status = _near(torch, t, dt, 1, 1e150)
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    full = device.compute("sakura", t, t, dt, 1, status=st)
+    torch.cuda.synchronize()
+    assert status == 0 and int(st.item()) == 0
+    for k in range(6):
+        a = full[k].cpu().numpy()
+        assert np.array_equal(a.view(np.uint64), got[k].view(np.uint64)), k
+
+
+def test_sakura_near_zero_cut_selects_nothing(torch):
+    d = _cluster(500, 3)
+    got, status = _near(torch, _dev(torch, d), 1.0 / 512, 1, 0.0)
+    assert status == 0
+    for k in range(6):
+        assert not np.any(got[k])

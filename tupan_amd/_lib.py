@@ -108,6 +108,8 @@ def lib():
         L.tupan_kick_pn_half_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
         L.tupan_pn_flux_dev.argtypes = [_SZ, _PP, _PP, _D, _P]
         L.tupan_apply_delta_dev.argtypes = [_SZ, _PP, _PP, _P]
+        L.tupan_pn_flux_kick_dev.argtypes = [_SZ, _PP, _PP, _PP, _D, _P]
+        L.tupan_advance_clock_dev.argtypes = [_SZ, _P, _P, _P, _D, _P]
         L.tupan_divide_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP,
                                        ctypes.c_double, _P]
         L.tupan_sum_dev.argtypes = [_SZ, _P, _P, _P]

@@ -112,6 +112,56 @@ struct SparsePairs<Pol, decltype((void)Pol::kSparse)> {
   static constexpr bool value = Pol::kSparse;
 };
 
+// Sparse policies' Params carry `mask` and `mask_tiles`: when the launcher
+// has filled a mask (near_mask_kernel), bit k of word
+// mask[w * mask_tiles + t] says whether some i of i-block w (64 P i's)
+// needs j = kTile t + k, and the engine reads one word per tile instead of
+// testing the tile's pairs itself.
+//
+// near_mask_kernel: one wave per (i-block, tile); lane 0 writes the word.  A separate
+// kernel because the sparse policies' pair() (a Kepler solve) needs several
+// times the registers of this test and would run it at a fraction of the
+// occupancy.
+template <class Pol, int P>
+__global__ __launch_bounds__(kBlock) void near_mask_kernel(
+    Soa is, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
+    int ntiles, typename Pol::Params prm, uint32_t *__restrict__ mask) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int iw = blockIdx.x;
+  const int i0 = iw * 64 * P;
+  const int t = blockIdx.y * kWaves + wave;
+  if (i0 >= ni || t >= ntiles) return;  // wave-uniform
+  typename Pol::I ist[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    int i = i0 + p * 64 + lane;
+    ist[p] = Pol::load_i(is, i < ni ? i : ni - 1, prm);
+  }
+  const int j0 = t * kTile;
+  const int nk = min(kTile, nj - j0);
+  uint32_t need = 0u;
+  if (nk == kTile) {
+#pragma unroll 8
+    for (int k = 0; k < kTile; ++k) {
+      const typename Pol::JRec r = jr[j0 + k];
+      bool nb = false;
+#pragma unroll
+      for (int p = 0; p < P; ++p) nb = nb || Pol::near(ist[p], r, prm);
+      need |= (__any(nb) ? 1u : 0u) << k;
+    }
+  } else {
+    for (int k = 0; k < nk; ++k) {
+      const typename Pol::JRec r = jr[j0 + k];
+      bool nb = false;
+#pragma unroll
+      for (int p = 0; p < P; ++p) nb = nb || Pol::near(ist[p], r, prm);
+      need |= (__any(nb) ? 1u : 0u) << k;
+    }
+  }
+  if (lane == 0) mask[(size_t)iw * ntiles + t] = need;
+}
+
 // ------------------------------------------------------------- engine
 // FP32C policies (kTwoLevel) also receive the caller's FP64 arrays and the
 // j index: a pair they classify as "near" is re-evaluated from those in FP64.
@@ -202,7 +252,9 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
         for (int j0 = jb; j0 < je; j0 += kTile) {
           const int nk = min(kTile, je - j0);
           uint32_t need = 0u;  // wave-uniform: bit k = some lane needs j0 + k
-          if (nk == kTile) {
+          if (prm.mask) {
+            need = prm.mask[(size_t)blockIdx.x * prm.mask_tiles + j0 / kTile];
+          } else if (nk == kTile) {
 #pragma unroll 8
             for (int k = 0; k < kTile; ++k) {
               const typename Pol::JRec r = jr[j0 + k];

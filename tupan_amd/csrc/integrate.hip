@@ -160,6 +160,53 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// pn_flux_kernel followed by the second half of kick_pn on the same
+// particle: the flux sees the velocity before the half kick, the half kick
+// the new w, as when the two kernels run one after the other.
+// p as pn_flux_kernel; q.f[0..2] = pn_amx..z (in/out), q.f[3..5] = ax..az
+__global__ void __launch_bounds__(kThreads)
+    pn_flux_kick_kernel(int n, P18 p, P18 q, double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+    const double m = p.f[0][i];
+    const double rx = p.f[1][i], ry = p.f[2][i], rz = p.f[3][i];
+    const double vx = p.f[4][i], vy = p.f[5][i], vz = p.f[6][i];
+    const double ax = p.f[7][i], ay = p.f[8][i], az = p.f[9][i];
+    const double fx = m * ax, fy = m * ay, fz = m * az;
+    p.f[13][i] = p.f[13][i] - (vx * fx + vy * fy + vz * fz) * tau;
+    p.f[14][i] = p.f[14][i] - fx * tau;
+    p.f[15][i] = p.f[15][i] - fy * tau;
+    p.f[16][i] = p.f[16][i] - fz * tau;
+    q.f[0][i] = q.f[0][i] - (ry * fz - rz * fy) * tau;
+    q.f[1][i] = q.f[1][i] - (rz * fx - rx * fz) * tau;
+    q.f[2][i] = q.f[2][i] - (rx * fy - ry * fx) * tau;
+    const double wx = 2 * ax * tau - p.f[10][i];
+    const double wy = 2 * ay * tau - p.f[11][i];
+    const double wz = 2 * az * tau - p.f[12][i];
+    p.f[10][i] = wx;
+    p.f[11][i] = wy;
+    p.f[12][i] = wz;
+    p.f[4][i] = vx + (q.f[3][i] * tau + wx) / 2;
+    p.f[5][i] = vy + (q.f[4][i] * tau + wy) / 2;
+    p.f[6][i] = vz + (q.f[5][i] * tau + wz) / 2;
+  }
+}
+
+// the end-of-step bookkeeping of a shared step: tstep = tau, time += tau,
+// nstep += 1
+__global__ void __launch_bounds__(kThreads)
+    advance_clock_kernel(int n, double *tstep, double *time,
+                         long long *nstep, double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+    tstep[i] = tau;
+    time[i] = time[i] + tau;
+    nstep[i] = nstep[i] + 1;
+  }
+}
+
 // x += d over six fields (r, v)
 __global__ void __launch_bounds__(kThreads)
     apply_delta_kernel(int n, P18 p) {
@@ -352,6 +399,42 @@ int tupan_pn_flux_dev(size_t n, const double *const *in, double *const *acc,
   hipLaunchKernelGGL(pn_flux_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
                      reinterpret_cast<hipStream_t>(stream), (int)n, p, q, tau);
   return finish("pn_flux");
+}
+
+int tupan_pn_flux_kick_dev(size_t n, double *const *in, double *const *acc,
+                           const double *const *a, double tau, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)in, 10, "input"))) return rc;
+  if ((rc = check_fields((const void *const *)acc, 10, "accumulator")))
+    return rc;
+  if ((rc = check_fields((const void *const *)a, 3, "acceleration")))
+    return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{}, q{};
+  for (int k = 0; k < 10; ++k) p.f[k] = in[k];
+  for (int k = 0; k < 7; ++k) p.f[10 + k] = acc[k];
+  for (int k = 0; k < 3; ++k) {
+    q.f[k] = acc[7 + k];
+    q.f[3 + k] = const_cast<double *>(a[k]);
+  }
+  hipLaunchKernelGGL(pn_flux_kick_kernel, dim3(grid_for(n)), dim3(kThreads),
+                     0, reinterpret_cast<hipStream_t>(stream), (int)n, p, q,
+                     tau);
+  return finish("pn_flux_kick");
+}
+
+int tupan_advance_clock_dev(size_t n, double *tstep, double *time,
+                            long long *nstep, double tau, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if (!tstep || !time || !nstep)
+    return set_error(TUPAN_ERR_ARG, "NULL clock field");
+  if (n == 0) return TUPAN_OK;
+  hipLaunchKernelGGL(advance_clock_kernel, dim3(grid_for(n)), dim3(kThreads),
+                     0, reinterpret_cast<hipStream_t>(stream), (int)n, tstep,
+                     time, nstep, tau);
+  return finish("advance_clock");
 }
 
 int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,

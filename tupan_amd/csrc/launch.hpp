@@ -26,7 +26,11 @@ void device_free(void *p, hipStream_t st);
 // entry points (no allocation in steady state; safe to capture).  Each
 // buffer in simultaneous use within one call needs its own slot.
 enum ScratchSlot { kSlotRecords = 0, kSlotPartials, kSlotIeee, kSlotReduce,
-                   kSlotPairwise };
+                   kSlotPairwise, kSlotMask };
+
+// Largest near mask (sparse policies) the launcher builds; above it the
+// engine tests the pairs itself.
+constexpr size_t kMaxMaskBytes = size_t(64) << 20;
 int scratch_alloc(int slot, void **p, size_t bytes, hipStream_t st);
 void scratch_free(int slot, void *p, hipStream_t st);
 // Optional live timing of the pair-engine launches (tupan_set_timing):
@@ -150,6 +154,28 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
                        dim3(256), 0, st, js, n_j, jr, (const double *)ctx);
   }
+  // sparse policies: the near mask, when it fits (P = 1 only)
+  typename Pol::Params prm2 = prm;
+  uint32_t *mask = nullptr;
+  if constexpr (SparsePairs<Pol>::value) {
+    const size_t mtiles = ((size_t)n_j + kTile - 1) / kTile;
+    const size_t mbytes = sizeof(uint32_t) * (size_t)waves * mtiles;
+    const size_t mrows = (mtiles + kWaves - 1) / kWaves;
+    if (P == 1 && n_j > 0 && Pol::sparse(prm) && mbytes <= kMaxMaskBytes &&
+        mrows <= 65535) {
+      rc = scratch_alloc(kSlotMask, (void **)&mask, mbytes, st);
+      if (rc) {
+        scratch_free(kSlotPartials, part, st);
+        scratch_free(kSlotRecords, jr, st);
+        return rc;
+      }
+      hipLaunchKernelGGL((near_mask_kernel<Pol, 1>),
+                         dim3((unsigned)waves, (unsigned)mrows), dim3(kBlock),
+                         0, st, is, n_i, jr, n_j, (int)mtiles, prm, mask);
+      prm2.mask = mask;
+      prm2.mask_tiles = (int)mtiles;
+    }
+  }
   dim3 grid((unsigned)waves, (unsigned)SB);
   const double *aux = ctx ? ctx + 2 : nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -158,20 +184,22 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   switch (P) {
     case 4:
       launch_p<Pol, (Pol::kMaxIpl >= 4 ? 4 : 1)>(grid, st, is, js, n_i, jr, n_j,
-                                                  jchunk, SB, part, out, prm, aux);
+                                                  jchunk, SB, part, out, prm2, aux);
       break;
     case 2:
       launch_p<Pol, (Pol::kMaxIpl >= 2 ? 2 : 1)>(grid, st, is, js, n_i, jr, n_j,
-                                                  jchunk, SB, part, out, prm, aux);
+                                                  jchunk, SB, part, out, prm2, aux);
       break;
     default:
-      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, SB, part, out, prm, aux);
+      launch_p<Pol, 1>(grid, st, is, js, n_i, jr, n_j, jchunk, SB, part, out,
+                       prm2, aux);
   }
   if (timed) (void)hipEventRecord(ev1, st);
   if (SB > 1)
     hipLaunchKernelGGL((finish_kernel<Pol>), dim3((n_i + 255) / 256),
                        dim3(256), 0, st, part, SB, n_i, out, prm);
   hipError_t e = hipGetLastError();
+  if (mask) scratch_free(kSlotMask, mask, st);
   scratch_free(kSlotPartials, part, st);
   scratch_free(kSlotRecords, jr, st);
   if (e != hipSuccess) return hip_fail(e, "kernel launch");

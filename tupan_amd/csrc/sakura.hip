@@ -18,6 +18,8 @@ struct SakuraParams {
   int budget;
   int *status;
   double rc2;  // only pairs with r^2 < rc2 evolve (infinity: all pairs)
+  const uint32_t *mask;  // filled by the launcher (near_mask_kernel)
+  int mask_tiles;
 };
 
 struct SakuraF64 : SumOut<6> {
@@ -38,7 +40,9 @@ struct SakuraF64 : SumOut<6> {
   __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   // Cut-off runs (finite rc2) test every pair before evolving any.
   static constexpr bool kSparse = true;
-  __device__ static bool sparse(const Params &prm) { return prm.rc2 < INFINITY; }
+  __host__ __device__ static bool sparse(const Params &prm) {
+    return prm.rc2 < INFINITY;
+  }
   // The pairs pair() evolves: r^2 < rc2, except the zero state (r = v = 0:
   // a particle paired with itself, or an exact duplicate) of a pair whose
   // weight m_j / m is finite, with a finite dt.  pair_evolve leaves the

@@ -336,10 +336,11 @@ class Base(object):
                                     "budget")
 
     def _advance_clock(self, ps, tau):
+        """ps.tstep = tau, ps.time += tau, ps.nstep += 1: one launch."""
         ps.t_curr += tau
-        ps.tstep.fill_(tau)
-        ps.time.add_(tau)
-        ps.nstep.add_(1)
+        _lib.call_update("advance_clock", ps.n, ps.tstep.data_ptr(),
+                         ps.time.data_ptr(), ps.nstep.data_ptr(), float(tau),
+                         _stream(ps))
 
 
 class Hermite(Base):
@@ -416,8 +417,14 @@ class SIA(Base):
             ps.ptrs(("pn_mvx", "pn_mvy", "pn_mvz")) if pn else None,
             _stream(ps))
 
-    def kick(self, ps, tau):
+    def newton_acc(self, ps):
         ps.set_acc()
+
+    def kick(self, ps, tau):
+        """kick_n, or kick_pn (sia.py:142-156): half kick with w, PN
+        accelerations at the half-kicked velocities, the flux bookkeeping
+        and the second half kick (the last two fused in one launch)."""
+        self.newton_acc(ps)
         v = ps.ptrs(("vx", "vy", "vz"))
         a = ps.ptrs(("ax", "ay", "az"))
         st = _stream(ps)
@@ -428,12 +435,11 @@ class SIA(Base):
         _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
         ps.set_pnacc(self.clight)
         _lib.call_update(
-            "pn_flux", ps.n,
+            "pn_flux_kick", ps.n,
             ps.ptrs(("mass", "rx", "ry", "rz", "vx", "vy", "vz",
                      "pnax", "pnay", "pnaz")),
             ps.ptrs(("wx", "wy", "wz", "pn_ke", "pn_mvx", "pn_mvy", "pn_mvz",
-                     "pn_amx", "pn_amy", "pn_amz")), float(tau), st)
-        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
+                     "pn_amx", "pn_amy", "pn_amz")), a, float(tau), st)
 
     def kdk(self, ps, tau):
         self.kick(ps, 0.5 * tau)
@@ -523,24 +529,9 @@ class SakuraPN(SIA):
         for f in PN_FIELDS:
             ps.field(f)
 
-    def kick(self, ps, tau):
+    def newton_acc(self, ps):
+        # the far pairs only (r >= rcut), in FP64 whatever the set's mode
         ps._pair("acc_far", self.rcut, mode="fp64")
-        v = ps.ptrs(("vx", "vy", "vz"))
-        a = ps.ptrs(("ax", "ay", "az"))
-        st = _stream(ps)
-        if not ps.include_pn_corrections:
-            _lib.call_update("kick", ps.n, v, a, float(tau), st)
-            return
-        w = ps.ptrs(("wx", "wy", "wz"))
-        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
-        ps.set_pnacc(self.clight)
-        _lib.call_update(
-            "pn_flux", ps.n,
-            ps.ptrs(("mass", "rx", "ry", "rz", "vx", "vy", "vz",
-                     "pnax", "pnay", "pnaz")),
-            ps.ptrs(("wx", "wy", "wz", "pn_ke", "pn_mvx", "pn_mvy", "pn_mvz",
-                     "pn_amx", "pn_amy", "pn_amz")), float(tau), st)
-        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
 
     def drift(self, ps, tau):
         r = ps.ptrs(("rx", "ry", "rz"))
