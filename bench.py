@@ -17,13 +17,17 @@ Reported:
   accuracy    per-particle |a - a64|/|a64| and |j - j64|/|j64| (max, rms)
               against the FP64 kernel on the same inputs
   fp64        throughput of the FP64 kernel on the same workload
+  host_abi    end-to-end rate through the host-pointer entry point (H2D +
+              kernel + D2H per call; never the value)
   fp64_ieee   throughput of the reference-arithmetic mode (the reference C's
               operations, order and rounding on the GPU), whose outputs on
               the cpu_baseline's sampled rows must equal the reference C's
               bit for bit
   cpu_baseline  the reference's C AccJerk (oracle/_ref, or our bit-identical
-              C restatement when _ref is absent), single-threaded, on a
-              sample of i-rows x all N j-particles
+              C restatement when _ref is absent) on a sample of i-rows x all
+              N j-particles, on the host threads this process may use (at
+              most 16; the reference itself is single-threaded, and that
+              figure is reported beside it)
 """
 import argparse
 import json
@@ -129,42 +133,92 @@ def time_steps(fn, steps, warmup, torch):
     return elapsed, k_ms / max(k_n, 1), k_n
 
 
-def cpu_baseline(ps, n, target_s, gpu_ieee=None):
-    """Reference C AccJerk on a bounded sample of i-rows.  gpu_ieee: host
-    copies of the GPU's fp64_ieee outputs for all n rows; the sample's
-    outputs are compared with them bit for bit."""
+def host_abi_rates(ps, n, torch, calls=5):
+    """End-to-end rate through the host-pointer entry point (the path of the
+    drop-in libtupan.h symbols): H2D of the 8 input fields from pageable
+    numpy arrays, the kernel, D2H of the 6 outputs, synchronous."""
+    from tupan_amd import kernels, spec
+    host = {f: np.ascontiguousarray(getattr(ps, f)) for f in spec.F8}
+    res = {"note": "tupan_acc_jerk_host: H2D 8 x %d FP64 (pageable), kernel, "
+                   "D2H 6 x %d, synchronous; %d calls" % (n, n, calls)}
+    for mode in ("fp32c", "fp64"):
+        kernels.compute("acc_jerk", host, host, mode=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            kernels.compute("acc_jerk", host, host, mode=mode)
+        dt = (time.perf_counter() - t0) / calls
+        res[mode] = {"value": float(n) * n / dt / 1e9,
+                     "unit": "Ginteractions/s", "ms_per_call": 1e3 * dt}
+    return res
+
+
+def host_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    at most 16 (one GPU's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(ps, n, target_s, gpu_ieee=None, threads=None):
+    """Reference C AccJerk on a bounded sample of i-rows, on one thread and
+    on ``threads`` host threads (the rows dealt in contiguous slices, one
+    C call per slice: ctypes drops the GIL for the call, and the C kernel is
+    reentrant).  gpu_ieee: host copies of the GPU's fp64_ieee outputs for
+    all n rows; the multi-thread sample's outputs are compared with them bit
+    for bit."""
+    from concurrent.futures import ThreadPoolExecutor
     import oracle
     kind = "reference" if oracle.available("float64", "reference") else "port"
     o = oracle.Oracle("float64", kind)
     fields = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
     jd = {f: np.ascontiguousarray(getattr(ps, f)) for f in fields}
     rng = np.random.RandomState(0)
+    threads = threads or host_threads()
 
-    def run(rows):
+    def run(rows, nthr):
         idx = np.sort(rng.choice(n, rows, replace=False))
-        idd = {f: v[idx] for f, v in jd.items()}
+        idd = {f: np.ascontiguousarray(v[idx]) for f, v in jd.items()}
+        cuts = np.linspace(0, rows, nthr + 1).astype(int)
+        parts = [{f: v[cuts[k]:cuts[k + 1]] for f, v in idd.items()}
+                 for k in range(nthr)]
         t0 = time.perf_counter()
-        res = o.call("acc_jerk_kernel", idd, jd)
-        return time.perf_counter() - t0, idx, res
+        if nthr == 1:
+            outs = [o.call("acc_jerk_kernel", parts[0], jd)]
+        else:
+            with ThreadPoolExecutor(nthr) as ex:
+                outs = list(ex.map(
+                    lambda q: o.call("acc_jerk_kernel", q, jd), parts))
+        dt = time.perf_counter() - t0
+        res = [np.concatenate([r[k] for r in outs]) for k in range(6)]
+        return dt, idx, res
 
     probe_rows = 64
-    dt, _, _ = run(probe_rows)
+    dt, _, _ = run(probe_rows, 1)
     rows = int(min(n, max(probe_rows, probe_rows * target_s / max(dt, 1e-6))))
-    dt, idx, res = run(rows)
-    rate = rows * n / dt
+    dt1, _, _ = run(rows, 1)
+    rate1 = rows * n / dt1
+    rows_mt = int(min(n, rows * threads))
+    dtm, idx, res = run(rows_mt, threads)
+    rate = rows_mt * n / dtm
     extra = {}
     if gpu_ieee is not None:
         same = all(np.array_equal(r.view(np.uint64), g[idx].view(np.uint64))
                    for r, g in zip(res, gpu_ieee))
         extra["fp64_ieee_bitwise_equal"] = bool(same)
-    return dict(extra, **{"value": rate / 1e9, "unit": "Ginteractions/s",
-                          "cores": 1,
-            "kind": kind,
-            "sample": "AccJerk FP64, %d sampled i-rows x all %d j (%.1f s, "
-                      "1 thread, %s)" % (rows, n, dt,
-                                         "reference C tupan/lib/src compiled "
-                                         "-O3 -std=c99" if kind == "reference"
-                                         else "bit-identical C restatement")})
+    what = ("reference C tupan/lib/src compiled -O3 -std=c99"
+            if kind == "reference" else "bit-identical C restatement")
+    return dict(extra, **{
+        "value": rate / 1e9, "unit": "Ginteractions/s", "cores": threads,
+        "kind": kind,
+        "sample": "AccJerk FP64, %d sampled i-rows x all %d j (%.1f s, %d "
+                  "host threads, %s)" % (rows_mt, n, dtm, threads, what),
+        "single_thread": {"value": rate1 / 1e9, "unit": "Ginteractions/s",
+                          "sample": "%d sampled i-rows x all %d j (%.1f s)"
+                                    % (rows, n, dt1)}})
 
 
 ENS_CANDIDATES = [(1, 256, 8), (1, 512, 8), (1, 1024, 8), (2, 512, 8),
@@ -708,6 +762,9 @@ def main():
         result["fp64_ieee"] = {"value": pairs * 3 / ei / 1e9,
                                "unit": "Ginteractions/s", "kernel_ms": ki}
         gpu_ieee = [t.cpu().numpy() for t in ieee]
+    if args.mode == "fp32c" and not args.no_fp64:
+        print("[bench] host-pointer ABI", file=sys.stderr, flush=True)
+        result["host_abi"] = host_abi_rates(ps, n, torch)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(ps, n, args.cpu_seconds,
