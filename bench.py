@@ -61,10 +61,11 @@ def parse():
                    help="pn workload: replay each step as a HIP graph")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--workload", default="accjerk",
-                   choices=["accjerk", "sizes", "ensemble", "pn", "diag",
+                   choices=["accjerk", "c1", "sizes", "ensemble", "pn", "diag",
                             "nreg", "adjoint"],
                    help="accjerk: the headline single-evaluation metric; "
-                        "sizes: Acc, AccJerk, SnapCrackle at N = 16K, 64K, "
+                        "c1: Hermite-4 steps at N = 256 per mode, eager and "
+                        "graph-replayed; sizes: Acc, AccJerk, SnapCrackle at N = 16K, 64K, "
                         "256K in both modes with errors vs FP64; "
                         "ensemble: the 512-cluster ragged ensemble (adaptive "
                         "Hermite-4) against looping the single-system path; "
@@ -570,6 +571,84 @@ def bench_nreg(args, torch):
     print(json.dumps(result))
 
 
+def bench_c1(args, torch):
+    """The Hermite-4 trajectory case: Plummer N=256, seed 0, equal masses,
+    eps^2 = 1e-4 (eps2 field 5e-5), P(EC)^2 with dt = 1/64: 64 steps per time
+    unit, each 3 AccJerk evaluations.  Per mode (fp64, fp32c, fp64_ieee),
+    eager and replayed as a HIP graph, one time unit is run to warm up
+    (scratch, capture) and the next one is timed.  Reference: the checker's
+    Hermite driver (hermite.py's update formulas in numpy) on the reference
+    C kernels, one host thread, for the first time unit; the fp64_ieee run's
+    state after it must equal the reference run's bit for bit."""
+    from tupan_amd import integrator as ig
+    from tupan_amd import kernels
+    from tupan_amd.ics import make_plummer
+    n, dt = 256, 1.0 / 64
+    base = ("mass", "eps2", "rx", "ry", "rz", "vx", "vy", "vz")
+    phi = lambda q: kernels.compute("phi", q, q, mode="fp64")[0]
+    p = make_plummer(n, 0.01, seed=0, phi_fn=phi)
+    init = {f: np.ascontiguousarray(getattr(p, f), dtype=np.float64)
+            for f in base}
+    runs, ieee_state = {}, None
+    for mode in ("fp64", "fp32c", "fp64_ieee"):
+        for graph in (False, True):
+            ps = ig.System.from_host(init)
+            integ = ig.Integrator(dt, 0.0, ps, "hermite4", mode=mode,
+                                  graph=graph)
+            integ.evolve(1.0)
+            if mode == "fp64_ieee" and not graph:
+                ieee_state = ps.to_host(base)
+            el = None
+            for unit in (2.0, 3.0):  # best of two timed units
+                torch.cuda.synchronize()
+                nst = int(ps.nstep[0].item())
+                t0 = time.perf_counter()
+                integ.evolve(unit)
+                torch.cuda.synchronize()
+                e = time.perf_counter() - t0
+                steps = int(ps.nstep[0].item()) - nst
+                el = e if el is None else min(el, e)
+            runs["%s%s" % (mode, "_graph" if graph else "")] = {
+                "steps_per_s": steps / el, "ms_per_step": 1e3 * el / steps}
+            print("[bench] c1 %s graph=%s: %.3f ms/step" % (
+                mode, graph, 1e3 * el / steps), file=sys.stderr, flush=True)
+    import oracle
+    from oracle import integrators as OI
+    kind = "reference" if oracle.available("float64", "reference") else "port"
+    K = OI.Kernels(oracle.Oracle("float64", kind))
+    host = {f: v.copy() for f, v in init.items()}
+    host["t_curr"] = 0.0
+    host["tstep"] = np.zeros(n)
+    t = time.perf_counter()
+    cpu_steps = 0
+    while abs(host["t_curr"]) < 1.0:
+        tau = OI.get_base_tstep(host["t_curr"], 1.0, dt)
+        OI.hermite_step(K, host, tau, 4)
+        cpu_steps += 1
+    cpu_el = time.perf_counter() - t
+    same = all(np.array_equal(ieee_state[f].view(np.uint64),
+                              host[f].view(np.uint64)) for f in base)
+    best = runs["fp64_graph"]
+    result = {
+        "metric": "Hermite-4 steps/s, Plummer N=256 (FP64)",
+        "value": best["steps_per_s"], "unit": "steps/s", "n_gpus": 1,
+        "steps": 64, "warmup": 128, "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": "hermite4 P(EC)^2, Plummer N=256 seed 0, "
+                               "equal masses, eps^2=1e-4, dt=1/64, one time "
+                               "unit (3 AccJerk evaluations per step)",
+                   "graph": True},
+        "modes": runs,
+        "fp64_ieee_bitwise_equal": bool(same),
+        "cpu_baseline": {"value": cpu_steps / cpu_el, "unit": "steps/s",
+                         "cores": 1, "kind": kind,
+                         "sample": "%d Hermite-4 steps (numpy driver + C "
+                                   "AccJerk), 1 thread" % cpu_steps},
+    }
+    print(json.dumps(result))
+
+
 def bench_adjoint(args, torch):
     """The differentiation workload: Plummer N=16384, seed 3, eps^2 = 1e-4,
     target field from the masses perturbed by a seeded 5% Gaussian,
@@ -680,6 +759,9 @@ def main():
         return
     if args.workload == "nreg":
         bench_nreg(args, torch)
+        return
+    if args.workload == "c1":
+        bench_c1(args, torch)
         return
     if args.workload == "adjoint":
         bench_adjoint(args, torch)
