@@ -114,52 +114,51 @@ struct SparsePairs<Pol, decltype((void)Pol::kSparse)> {
 
 // Sparse policies' Params carry `mask` and `mask_tiles`: when the launcher
 // has filled a mask (near_mask_kernel), bit k of word
-// mask[w * mask_tiles + t] says whether some i of i-block w (64 P i's)
-// needs j = kTile t + k, and the engine reads one word per tile instead of
-// testing the tile's pairs itself.
+// mask[w * mask_tiles + t] is set when some i of i-block w (64 i's, P = 1)
+// may need j = kTile t + k, and the engine reads one word per tile instead
+// of testing the tile's pairs itself.  The mask may hold extra bits (it
+// is a superset: pair() applies the exact test), never fewer.
 //
-// near_mask_kernel: one wave per (i-block, tile); lane 0 writes the word.  A separate
-// kernel because the sparse policies' pair() (a Kepler solve) needs several
-// times the registers of this test and would run it at a fraction of the
-// occupancy.
-template <class Pol, int P>
+// near_mask_kernel: lanes own j's (two 32-j tiles per wave, positions in
+// VGPRs) and walk the i-block's 64 i's, whose positions are wave-uniform
+// scalar loads; bit = r^2 < Pol::mask_cut(prm), a cut-off slightly above
+// the policy's own so that a different rounding of r^2 cannot drop a pair.
+// A separate kernel because the sparse policies' pair() (a Kepler solve)
+// needs several times the registers of this test and would run it at a
+// fraction of the occupancy.
+// `same`: the i and j sets are the same arrays, so j == i is the particle
+// paired with itself; where Pol::self_noop says that pair adds exactly
+// nothing, it gets no bit.
+template <class Pol>
 __global__ __launch_bounds__(kBlock) void near_mask_kernel(
-    Soa is, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
-    int ntiles, typename Pol::Params prm, uint32_t *__restrict__ mask) {
+    Soa is, int ni, Soa js, int nj, int ntiles, typename Pol::Params prm,
+    int same, uint32_t *__restrict__ mask) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int iw = blockIdx.x;
-  const int i0 = iw * 64 * P;
-  const int t = blockIdx.y * kWaves + wave;
-  if (i0 >= ni || t >= ntiles) return;  // wave-uniform
-  typename Pol::I ist[P];
-#pragma unroll
-  for (int p = 0; p < P; ++p) {
-    int i = i0 + p * 64 + lane;
-    ist[p] = Pol::load_i(is, i < ni ? i : ni - 1, prm);
-  }
-  const int j0 = t * kTile;
-  const int nk = min(kTile, nj - j0);
-  uint32_t need = 0u;
-  if (nk == kTile) {
+  const int i0 = iw * 64;
+  const int j0 = (blockIdx.y * kWaves + wave) * 64;
+  if (i0 >= ni || j0 >= nj) return;  // wave-uniform
+  const int j = min(j0 + lane, nj - 1);
+  const double xj = js.f[RX][j], yj = js.f[RY][j], zj = js.f[RZ][j];
+  const double cut = Pol::mask_cut(prm);
+  // the lane's own index when its self pair is skipped, else -1
+  const int self = same && Pol::self_noop(js, j, prm) ? j : -1;
+  const int ie = min(ni, i0 + 64);
+  bool nb = false;
 #pragma unroll 8
-    for (int k = 0; k < kTile; ++k) {
-      const typename Pol::JRec r = jr[j0 + k];
-      bool nb = false;
-#pragma unroll
-      for (int p = 0; p < P; ++p) nb = nb || Pol::near(ist[p], r, prm);
-      need |= (__any(nb) ? 1u : 0u) << k;
-    }
-  } else {
-    for (int k = 0; k < nk; ++k) {
-      const typename Pol::JRec r = jr[j0 + k];
-      bool nb = false;
-#pragma unroll
-      for (int p = 0; p < P; ++p) nb = nb || Pol::near(ist[p], r, prm);
-      need |= (__any(nb) ? 1u : 0u) << k;
-    }
+  for (int i = i0; i < ie; ++i) {
+    const double dx = is.f[RX][i] - xj;
+    const double dy = is.f[RY][i] - yj;
+    const double dz = is.f[RZ][i] - zj;
+    nb |= (dx * dx + dy * dy + dz * dz < cut) & (i != self);
   }
-  if (lane == 0) mask[(size_t)iw * ntiles + t] = need;
+  const uint64_t bits = __builtin_amdgcn_ballot_w64(nb && j0 + lane < nj);
+  const int t = j0 / kTile;
+  if (lane == 0) {
+    mask[(size_t)iw * ntiles + t] = (uint32_t)bits;
+    if (t + 1 < ntiles) mask[(size_t)iw * ntiles + t + 1] = (uint32_t)(bits >> 32);
+  }
 }
 
 // ------------------------------------------------------------- engine

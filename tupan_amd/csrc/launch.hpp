@@ -160,7 +160,8 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   if constexpr (SparsePairs<Pol>::value) {
     const size_t mtiles = ((size_t)n_j + kTile - 1) / kTile;
     const size_t mbytes = sizeof(uint32_t) * (size_t)waves * mtiles;
-    const size_t mrows = (mtiles + kWaves - 1) / kWaves;
+    // one wave per (i-block, 64 j's), kWaves per block
+    const size_t mrows = ((size_t)n_j + 64 * kWaves - 1) / (64 * kWaves);
     if (P == 1 && n_j > 0 && Pol::sparse(prm) && mbytes <= kMaxMaskBytes &&
         mrows <= 65535) {
       rc = scratch_alloc(kSlotMask, (void **)&mask, mbytes, st);
@@ -169,9 +170,12 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
         scratch_free(kSlotRecords, jr, st);
         return rc;
       }
-      hipLaunchKernelGGL((near_mask_kernel<Pol, 1>),
+      int same = 1;
+      for (int k = 0; k < Pol::kFields; ++k) same &= is.f[k] == js.f[k];
+      hipLaunchKernelGGL((near_mask_kernel<Pol>),
                          dim3((unsigned)waves, (unsigned)mrows), dim3(kBlock),
-                         0, st, is, n_i, jr, n_j, (int)mtiles, prm, mask);
+                         0, st, is, n_i, js, n_j, (int)mtiles, prm, same,
+                         mask);
       prm2.mask = mask;
       prm2.mask_tiles = (int)mtiles;
     }

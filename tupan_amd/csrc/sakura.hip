@@ -43,6 +43,17 @@ struct SakuraF64 : SumOut<6> {
   __host__ __device__ static bool sparse(const Params &prm) {
     return prm.rc2 < INFINITY;
   }
+  // the near mask's cut-off on r^2: rc2 widened by 2^-30 (r^2 computed
+  // with another rounding differs from pair()'s by a few ulps)
+  __device__ static double mask_cut(const Params &prm) {
+    return prm.rc2 * (1.0 + 0x1p-30);
+  }
+  // particle j paired with itself is the zero state; it adds nothing when
+  // its weight and dt are finite (zero_state below)
+  __device__ static bool self_noop(const Soa &s, int j, const Params &prm) {
+    const double m = s.f[M][j] + s.f[M][j];
+    return m > 0 && m < INFINITY && fabs(prm.dt) < INFINITY;
+  }
   // The pairs pair() evolves: r^2 < rc2, except the zero state (r = v = 0:
   // a particle paired with itself, or an exact duplicate) of a pair whose
   // weight m_j / m is finite, with a finite dt.  pair_evolve leaves the
