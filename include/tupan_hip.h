@@ -195,6 +195,46 @@ int tupan_nreg_v_host(int mode, size_t ni, const double *const *ip,
                       size_t nj, const double *const *jp, double dt,
                       double *const *op);
 
+/* ---- NREG with its scalars in device memory (no reference counterpart:
+ * the reference keeps E0, W, U, S on the host, nreg.py:22-163).  A step
+ * runs without host round trips, so it can be captured into a HIP graph.
+ * The scalar block holds TUPAN_NREG_SCALARS doubles at these slots: */
+#define TUPAN_NREG_E0 0   /* initial energy */
+#define TUPAN_NREG_W 1    /* W, U, S of nreg.py */
+#define TUPAN_NREG_U 2
+#define TUPAN_NREG_S 3
+#define TUPAN_NREG_MTOT 4 /* total mass */
+#define TUPAN_NREG_DT 5   /* the current sub-step length */
+#define TUPAN_NREG_T 6    /* physical time */
+#define TUPAN_NREG_T0 7   /* physical time at the start of the step */
+#define TUPAN_NREG_SUM 8  /* a sum written by tupan_sum_pairwise_dev */
+#define TUPAN_NREG_HSUB 9 /* the regularised length of the anreg step */
+#define TUPAN_NREG_SCALARS 10
+/* tupan_nreg_x_dev / tupan_nreg_v_dev with dt read from device memory
+ * (d_dt, e.g. sc + TUPAN_NREG_DT) when the kernels start. */
+int tupan_nreg_x_devdt(int mode, size_t ni, const double *const *ip,
+                       size_t nj, const double *const *jp, const double *d_dt,
+                       double *const *op, void *stream);
+int tupan_nreg_v_devdt(int mode, size_t ni, const double *const *ip,
+                       size_t nj, const double *const *jp, const double *d_dt,
+                       double *const *op, void *stream);
+/* One scalar recurrence of nreg.py on the block sc (one thread, double
+ * arithmetic without contraction, as Python evaluates it): */
+#define TUPAN_NREG_OP_BEGIN 0 /* T0 = T                        (:154, :158) */
+#define TUPAN_NREG_OP_H 1     /* HSUB = h       (anreg: h = tau/2, :155)   */
+#define TUPAN_NREG_OP_HS 2    /* HSUB = 0.5*(h*S)              (:92, :94)  */
+#define TUPAN_NREG_OP_XDT 3   /* DT = 0.5*(HSUB/W)             (:81, :83)  */
+#define TUPAN_NREG_OP_VDT 4   /* DT = HSUB/U                   (:82)       */
+#define TUPAN_NREG_OP_XEND 5  /* U = 0.5*SUM; T = T + DT       (:31-33)    */
+#define TUPAN_NREG_OP_VEND 6  /* W = 0.25*SUM/MTOT - E0        (:57-58)    */
+#define TUPAN_NREG_OP_S 7     /* S = 1/(2/W - 1/S)             (:93)       */
+int tupan_nreg_scalars_dev(double *sc, int op, double h, void *stream);
+/* end of an NREG step (nreg.py:162-166): tstep[i] = T - T0, time[i] += tau,
+ * nstep[i] += 1. */
+int tupan_nreg_step_end_dev(size_t n, double *tstep, double *time,
+                            long long *nstep, const double *sc, double tau,
+                            void *stream);
+
 /* ---- Kepler: replaces kepler_solver_kernel, libtupan.h:231-246 -------
  * ip: F8 arrays of which particles 0 and 1 are used; op: rx, ry, rz, vx,
  * vy, vz of the two particles after dt (may alias ip's fields, as the
@@ -338,6 +378,10 @@ int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
  * c < k (1 <= k <= 9), correctly rounded; dst may alias src. */
 int tupan_divide_dev(size_t n, int k, const double *const *src,
                      double *const *dst, double s, void *stream);
+/* the same with the divisor read from device memory (no reference
+ * counterpart: NREG with device-resident scalars). */
+int tupan_divide_by_dev(size_t n, int k, const double *const *src,
+                        double *const *dst, const double *d_s, void *stream);
 
 /* ---- reductions ---------------------------------------------------------
  * No counterpart in the reference (it sums with numpy on the host).

@@ -8,6 +8,11 @@ the graph run's state equals the eager run's bit for bit -- the first step
 (eager on the capture stream), the second (captured and replayed) and the
 later replays alike.  The last step of each run is shorter (t_end is not a
 multiple of the step), so that step runs eagerly beside the graphs.
+
+NREG (nreg, anreg) replays with its scalars (E0, W, U, S, the physical
+clock) in device memory: after every step its state and physical time equal
+the host-driven run's bit for bit, and in the reference-arithmetic mode the
+checker's NREG on the reference's C kernels.
 """
 import numpy as np
 import pytest
@@ -89,6 +94,54 @@ def test_graph_replay_equals_eager(torch, method, kw, tau, compact):
 def test_graph_rejected_for_adaptive_methods(torch):
     from tupan_amd.integrator import Integrator, System
     s = System.from_host(_ics(32, 3, 0.01))
-    for method in ("ahermite4", "sia21a.kdk", "asakura", "nreg"):
+    for method in ("ahermite4", "sia21a.kdk", "asakura"):
         with pytest.raises(ValueError):
             Integrator(1.0 / 64, 0.0, s, method, graph=True)
+
+
+@pytest.mark.parametrize("method,mode", [("anreg", "fp64"), ("nreg", "fp64"),
+                                         ("nreg", "fp64_ieee")])
+def test_nreg_graph_equals_host_driven(torch, method, mode):
+    from tupan_amd.integrator import Integrator, System
+    d = _ics(200, 14, 1e-3)
+    host_s, graph_s = System.from_host(d), System.from_host(d)
+    host = Integrator(1.0 / 32, 0.0, host_s, method, mode=mode)
+    graph = Integrator(1.0 / 32, 0.0, graph_s, method, mode=mode, graph=True)
+    for k in range(6):
+        print("[graphs] %s %s step %d" % (method, mode, k), flush=True)
+        host.evolve_step(10.0)
+        graph.evolve_step(10.0)
+        graph_s.diagnostics()  # eager work between replays
+        a, b = host_s.to_host(STATE), graph_s.to_host(STATE)
+        for f in STATE:
+            assert np.array_equal(a[f].view(np.uint64),
+                                  b[f].view(np.uint64)), (k, f)
+        assert graph_s.t_curr == host_s.t_curr, k
+        for f in ("tstep", "time"):
+            assert np.array_equal(getattr(graph_s, f).cpu().numpy(),
+                                  getattr(host_s, f).cpu().numpy()), (k, f)
+        assert int(graph_s.nstep[0]) == k + 1
+    assert len(graph.integrator._graphs) == 1
+
+
+def test_nreg_graph_bitwise_vs_checker_in_ieee_mode(torch):
+    from oracle import Oracle
+    from oracle import integrators as oi
+    from tupan_amd.integrator import Integrator, System
+    K = oi.Kernels(Oracle("float64", "port"))
+    d = _ics(128, 15, 1e-3)
+    s = System.from_host(d)
+    integ = Integrator(1.0 / 32, 0.0, s, "anreg", mode="fp64_ieee",
+                       graph=True)
+    host = dict(d)
+    host["t_curr"] = 0.0
+    st = {}
+    oi.nreg_initialize(K, host, st)
+    for k in range(5):
+        integ.evolve_step(10.0)
+        oi.nreg_do_step(K, host, st, 1.0 / 32, "anreg")
+        got = s.to_host(STATE)
+        for f in STATE:
+            assert np.array_equal(got[f].view(np.uint64),
+                                  host[f].view(np.uint64)), (k, f)
+        assert s.t_curr == host["t_curr"], k

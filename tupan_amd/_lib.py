@@ -16,6 +16,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TUPAN_HIP_LIB",
                           os.path.join(HERE, "lib", "libtupan_hip.so"))
 
+# NREG scalar block slots and ops (include/tupan_hip.h)
+NREG_E0, NREG_W, NREG_U, NREG_S, NREG_MTOT = 0, 1, 2, 3, 4
+NREG_DT, NREG_T, NREG_T0, NREG_SUM, NREG_HSUB = 5, 6, 7, 8, 9
+NREG_SCALARS = 10
+(NREG_OP_BEGIN, NREG_OP_H, NREG_OP_HS, NREG_OP_XDT, NREG_OP_VDT,
+ NREG_OP_XEND, NREG_OP_VEND, NREG_OP_S) = range(8)
+
 MODE_FP64 = 0
 MODE_FP32C = 1
 MODE_FP64_IEEE = 2
@@ -112,6 +119,11 @@ def lib():
         L.tupan_advance_clock_dev.argtypes = [_SZ, _P, _P, _P, _D, _P]
         L.tupan_divide_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP,
                                        ctypes.c_double, _P]
+        L.tupan_divide_by_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP, _P, _P]
+        L.tupan_nreg_scalars_dev.argtypes = [_P, ctypes.c_int,
+                                             ctypes.c_double, _P]
+        L.tupan_nreg_step_end_dev.argtypes = [_SZ, _P, _P, _P, _P,
+                                              ctypes.c_double, _P]
         L.tupan_sum_dev.argtypes = [_SZ, _P, _P, _P]
         L.tupan_sum_pairwise_dev.argtypes = [_SZ, _P, _P, _P]
         L.tupan_energy_terms_dev.argtypes = [_SZ, _PP, _P, _P, _P]
@@ -120,6 +132,8 @@ def lib():
         for name in ("nreg_x", "nreg_v"):
             getattr(L, "tupan_%s_dev" % name).argtypes = [
                 _I, _SZ, _PP, _SZ, _PP, _D, _PP, _P]
+            getattr(L, "tupan_%s_devdt" % name).argtypes = [
+                _I, _SZ, _PP, _SZ, _PP, _P, _PP, _P]
             getattr(L, "tupan_%s_host" % name).argtypes = [
                 _I, _SZ, _PP, _SZ, _PP, _D, _PP]
         L.tupan_kepler_dev.argtypes = [_PP, _D, _PP, _P, _P]

@@ -161,6 +161,23 @@ __global__ __launch_bounds__(kBlock) void near_mask_kernel(
   }
 }
 
+// Policies whose parameters live partly in device memory (a scalar written
+// by an earlier kernel of the same stream) provide prepare(prm), which the
+// kernels call once at their start to resolve them into the by-value copy.
+template <class Pol, class = void>
+struct HasPrepare {
+  static constexpr bool value = false;
+};
+template <class Pol>
+struct HasPrepare<Pol, decltype((void)Pol::prepare(
+                           *(typename Pol::Params *)nullptr))> {
+  static constexpr bool value = true;
+};
+template <class Pol>
+__device__ __forceinline__ void prepare_params(typename Pol::Params &prm) {
+  if constexpr (HasPrepare<Pol>::value) prm = Pol::prepare(prm);
+}
+
 // ------------------------------------------------------------- engine
 // FP32C policies (kTwoLevel) also receive the caller's FP64 arrays and the
 // j index: a pair they classify as "near" is re-evaluated from those in FP64.
@@ -169,6 +186,7 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
     Soa is, Soa js, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
     int jchunk, int nrows, double *__restrict__ part, Out out,
     typename Pol::Params prm, const double *__restrict__ aux) {
+  prepare_params<Pol>(prm);
   const int lane = threadIdx.x & 63;
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // j range, and so the j-record loads, scalar

@@ -64,6 +64,7 @@ __global__ __launch_bounds__(256) void ieee_engine(
   constexpr int kPer = (kIeeeTile * kRecChunks + 255) / 256;
   __shared__ __align__(16) JRec tile[2][kIeeeTile];
 
+  prepare_params<Pol>(prm);
   const int lane = threadIdx.x & 63;
   const int i0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
   const bool active = i0 < ni;  // wave-uniform

@@ -207,6 +207,21 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// NREG's end of step from the scalar block: tstep = the physical time
+// advanced (T - T0), time += tau, nstep += 1
+__global__ void __launch_bounds__(kThreads)
+    nreg_step_end_kernel(int n, double *tstep, double *time,
+                         long long *nstep, const double *sc, double tau) {
+#pragma clang fp contract(off)
+  const double dt = sc[TUPAN_NREG_T] - sc[TUPAN_NREG_T0];
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+    tstep[i] = dt;
+    time[i] = time[i] + tau;
+    nstep[i] = nstep[i] + 1;
+  }
+}
+
 // x += d over six fields (r, v)
 __global__ void __launch_bounds__(kThreads)
     apply_delta_kernel(int n, P18 p) {
@@ -220,8 +235,9 @@ __global__ void __launch_bounds__(kThreads)
 
 // dst[k][i] = src[k][i] / s for k < nk (correctly rounded division)
 __global__ void __launch_bounds__(kThreads)
-    divide_kernel(int n, int nk, P18 p, double s) {
+    divide_kernel(int n, int nk, P18 p, double s, const double *sp) {
 #pragma clang fp contract(off)
+  if (sp) s = *sp;
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
        i += gridDim.x * kThreads) {
     for (int k = 0; k < nk; ++k) p.f[9 + k][i] = p.f[k][i] / s;
@@ -437,6 +453,20 @@ int tupan_advance_clock_dev(size_t n, double *tstep, double *time,
   return finish("advance_clock");
 }
 
+int tupan_nreg_step_end_dev(size_t n, double *tstep, double *time,
+                            long long *nstep, const double *sc, double tau,
+                            void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if (!tstep || !time || !nstep || !sc)
+    return set_error(TUPAN_ERR_ARG, "NULL clock field or scalar block");
+  if (n == 0) return TUPAN_OK;
+  hipLaunchKernelGGL(nreg_step_end_kernel, dim3(grid_for(n)), dim3(kThreads),
+                     0, reinterpret_cast<hipStream_t>(stream), (int)n, tstep,
+                     time, nstep, sc, tau);
+  return finish("nreg_step_end");
+}
+
 int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
                           void *stream) {
   int rc;
@@ -454,8 +484,9 @@ int tupan_apply_delta_dev(size_t n, double *const *x, const double *const *d,
   return finish("apply_delta");
 }
 
-int tupan_divide_dev(size_t n, int k, const double *const *src,
-                     double *const *dst, double s, void *stream) {
+static int divide(size_t n, int k, const double *const *src,
+                  double *const *dst, double s, const double *sp,
+                  void *stream) {
   int rc;
   if ((rc = check_n(n))) return rc;
   if (k < 1 || k > 9) return set_error(TUPAN_ERR_ARG, "k must be in 1..9");
@@ -469,8 +500,20 @@ int tupan_divide_dev(size_t n, int k, const double *const *src,
     p.f[9 + c] = dst[c];
   }
   hipLaunchKernelGGL(divide_kernel, dim3(grid_for(n)), dim3(kThreads), 0,
-                     reinterpret_cast<hipStream_t>(stream), (int)n, k, p, s);
+                     reinterpret_cast<hipStream_t>(stream), (int)n, k, p, s,
+                     sp);
   return finish("divide");
+}
+
+int tupan_divide_dev(size_t n, int k, const double *const *src,
+                     double *const *dst, double s, void *stream) {
+  return divide(n, k, src, dst, s, nullptr, stream);
+}
+
+int tupan_divide_by_dev(size_t n, int k, const double *const *src,
+                        double *const *dst, const double *d_s, void *stream) {
+  if (!d_s) return set_error(TUPAN_ERR_ARG, "NULL device divisor");
+  return divide(n, k, src, dst, 0.0, d_s, stream);
 }
 
 }  // extern "C"

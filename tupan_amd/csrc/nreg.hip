@@ -19,10 +19,19 @@ namespace tupan {
 
 struct NregParams {
   double dt;
-  const double *im;  // i-side masses, for the finishing factor
+  const double *im;   // i-side masses, for the finishing factor
+  const double *dtp;  // when set, dt is read from here at kernel start
 };
 
-struct NregXF64 {
+// dt from device memory when the caller passed it there
+struct NregDt {
+  __device__ static NregParams prepare(NregParams p) {
+    if (p.dtp) p.dt = *p.dtp;
+    return p;
+  }
+};
+
+struct NregXF64 : NregDt {
   static constexpr bool kTwoLevel = false;
   static constexpr int kFields = 8, kMaxIpl = 2, kPrefIpl = 1, kOut = 7;
   using Params = NregParams;
@@ -69,7 +78,7 @@ struct NregXF64 {
 };
 
 // Fields (reference order, libtupan.h:181-197): m, vx, vy, vz, ax, ay, az.
-struct NregVF64 {
+struct NregVF64 : NregDt {
   static constexpr bool kTwoLevel = false;
   static constexpr int kFields = 7, kMaxIpl = 2, kPrefIpl = 1, kOut = 4;
   using Params = NregParams;
@@ -111,7 +120,7 @@ struct NregVF64 {
 // The same sums in the reference's arithmetic (TUPAN_MODE_FP64_IEEE): its
 // operations in its order without contraction, correctly rounded division
 // and sqrt, one accumulator per i in ascending j order.
-struct NregXIeee : ieee::IeeeBase<7> {
+struct NregXIeee : ieee::IeeeBase<7>, NregDt {
   static constexpr int kFields = 8;
   using Params = NregParams;
   using JRec = NregXF64::JRec;
@@ -163,7 +172,7 @@ struct NregXIeee : ieee::IeeeBase<7> {
   }
 };
 
-struct NregVIeee : ieee::IeeeBase<4> {
+struct NregVIeee : ieee::IeeeBase<4>, NregDt {
   static constexpr int kFields = 7;
   using Params = NregParams;
   using JRec = NregVF64::JRec;
@@ -239,11 +248,61 @@ __global__ void kepler_kernel(Soa s, double dt, double *r1x, double *r1y,
   }
 }
 
+// One step of NREG's scalar recurrences (nreg.py:22-97) on the scalar
+// block (TUPAN_NREG_* slots), in Python's double arithmetic: one operation
+// per statement, no contraction, correctly rounded division.
+__global__ void nreg_scalar_kernel(double *sc, int op, double h) {
+#pragma clang fp contract(off)
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  switch (op) {
+    case TUPAN_NREG_OP_BEGIN:
+      sc[TUPAN_NREG_T0] = sc[TUPAN_NREG_T];
+      break;
+    case TUPAN_NREG_OP_H:
+      sc[TUPAN_NREG_HSUB] = h;
+      break;
+    case TUPAN_NREG_OP_HS:
+      sc[TUPAN_NREG_HSUB] = 0.5 * (h * sc[TUPAN_NREG_S]);
+      break;
+    case TUPAN_NREG_OP_XDT:
+      sc[TUPAN_NREG_DT] = 0.5 * (sc[TUPAN_NREG_HSUB] / sc[TUPAN_NREG_W]);
+      break;
+    case TUPAN_NREG_OP_VDT:
+      sc[TUPAN_NREG_DT] = sc[TUPAN_NREG_HSUB] / sc[TUPAN_NREG_U];
+      break;
+    case TUPAN_NREG_OP_XEND:
+      sc[TUPAN_NREG_U] = 0.5 * sc[TUPAN_NREG_SUM];
+      sc[TUPAN_NREG_T] = sc[TUPAN_NREG_T] + sc[TUPAN_NREG_DT];
+      break;
+    case TUPAN_NREG_OP_VEND: {
+      const double kin = 0.25 * sc[TUPAN_NREG_SUM] / sc[TUPAN_NREG_MTOT];
+      sc[TUPAN_NREG_W] = kin - sc[TUPAN_NREG_E0];
+      break;
+    }
+    case TUPAN_NREG_OP_S:
+      sc[TUPAN_NREG_S] =
+          1 / (2 / sc[TUPAN_NREG_W] - 1 / sc[TUPAN_NREG_S]);
+      break;
+    default:
+      break;
+  }
+}
+
 }  // namespace tupan
 
 using namespace tupan;
 
 extern "C" {
+
+int tupan_nreg_scalars_dev(double *sc, int op, double h, void *stream) {
+  if (!sc) return set_error(TUPAN_ERR_ARG, "NULL scalar block");
+  if (op < TUPAN_NREG_OP_BEGIN || op > TUPAN_NREG_OP_S)
+    return set_error(TUPAN_ERR_ARG, "unknown NREG scalar op %d", op);
+  hipLaunchKernelGGL(nreg_scalar_kernel, dim3(1), dim3(64), 0,
+                     reinterpret_cast<hipStream_t>(stream), sc, op, h);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TUPAN_OK : hip_fail(e, "nreg scalar launch");
+}
 
 static int nreg_mode_ok(int mode) {
   if (mode == TUPAN_MODE_FP64 || mode == TUPAN_MODE_FP32C ||
@@ -252,18 +311,45 @@ static int nreg_mode_ok(int mode) {
   return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
 }
 
-int tupan_nreg_x_dev(int mode, size_t ni, const double *const *ip, size_t nj,
-                     const double *const *jp, double dt, double *const *op,
-                     void *stream) {
+static int nreg_x(int mode, size_t ni, const double *const *ip, size_t nj,
+                  const double *const *jp, double dt, const double *dtp,
+                  double *const *op, void *stream) {
   if (int rc = nreg_mode_ok(mode)) return rc;
   if (ni && (!ip || !ip[M]))
     return set_error(TUPAN_ERR_ARG, "NULL i mass field");
-  NregParams prm{dt, ni ? ip[M] : nullptr};
+  NregParams prm{dt, ni ? ip[M] : nullptr, dtp};
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   if (mode == TUPAN_MODE_FP64_IEEE)
     return ieee::run_ieee_engine<NregXIeee, 4>(ni, ip, nj, jp, op, st, prm);
   return run_engine<NregXF64>(TUPAN_K_NREG_X, TUPAN_MODE_FP64, ni, ip, nj,
                               jp, op, prm, st);
+}
+
+static int nreg_v(int mode, size_t ni, const double *const *ip, size_t nj,
+                  const double *const *jp, double dt, const double *dtp,
+                  double *const *op, void *stream) {
+  if (int rc = nreg_mode_ok(mode)) return rc;
+  if (ni && (!ip || !ip[0]))
+    return set_error(TUPAN_ERR_ARG, "NULL i mass field");
+  NregParams prm{dt, ni ? ip[0] : nullptr, dtp};
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if (mode == TUPAN_MODE_FP64_IEEE)
+    return ieee::run_ieee_engine<NregVIeee, 4>(ni, ip, nj, jp, op, st, prm);
+  return run_engine<NregVF64>(TUPAN_K_NREG_V, TUPAN_MODE_FP64, ni, ip, nj,
+                              jp, op, prm, st);
+}
+
+int tupan_nreg_x_dev(int mode, size_t ni, const double *const *ip, size_t nj,
+                     const double *const *jp, double dt, double *const *op,
+                     void *stream) {
+  return nreg_x(mode, ni, ip, nj, jp, dt, nullptr, op, stream);
+}
+
+int tupan_nreg_x_devdt(int mode, size_t ni, const double *const *ip,
+                       size_t nj, const double *const *jp, const double *d_dt,
+                       double *const *op, void *stream) {
+  if (!d_dt) return set_error(TUPAN_ERR_ARG, "NULL device dt");
+  return nreg_x(mode, ni, ip, nj, jp, 0.0, d_dt, op, stream);
 }
 
 int tupan_nreg_x_host(int mode, size_t ni, const double *const *ip,
@@ -280,15 +366,14 @@ int tupan_nreg_x_host(int mode, size_t ni, const double *const *ip,
 int tupan_nreg_v_dev(int mode, size_t ni, const double *const *ip, size_t nj,
                      const double *const *jp, double dt, double *const *op,
                      void *stream) {
-  if (int rc = nreg_mode_ok(mode)) return rc;
-  if (ni && (!ip || !ip[0]))
-    return set_error(TUPAN_ERR_ARG, "NULL i mass field");
-  NregParams prm{dt, ni ? ip[0] : nullptr};
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if (mode == TUPAN_MODE_FP64_IEEE)
-    return ieee::run_ieee_engine<NregVIeee, 4>(ni, ip, nj, jp, op, st, prm);
-  return run_engine<NregVF64>(TUPAN_K_NREG_V, TUPAN_MODE_FP64, ni, ip, nj,
-                              jp, op, prm, st);
+  return nreg_v(mode, ni, ip, nj, jp, dt, nullptr, op, stream);
+}
+
+int tupan_nreg_v_devdt(int mode, size_t ni, const double *const *ip,
+                       size_t nj, const double *const *jp, const double *d_dt,
+                       double *const *op, void *stream) {
+  if (!d_dt) return set_error(TUPAN_ERR_ARG, "NULL device dt");
+  return nreg_v(mode, ni, ip, nj, jp, 0.0, d_dt, op, stream);
 }
 
 int tupan_nreg_v_host(int mode, size_t ni, const double *const *ip,

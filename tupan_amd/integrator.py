@@ -306,19 +306,29 @@ class Base(object):
             self._gstream.wait_stream(cur)
             if tau not in self._seen:
                 with torch.cuda.stream(self._gstream):
-                    ps = self.do_step(ps, tau)
+                    ps = self._eager_graph_step(ps, tau)
                 cur.wait_stream(self._gstream)
                 self._seen.add(tau)
                 return ps
             t0 = ps.t_curr
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g, stream=self._gstream):
-                self.do_step(ps, tau)
+                self._capture_body(ps, tau)
             ps.t_curr = t0
             self._graphs[tau] = g
         g.replay()
-        ps.t_curr += tau
+        self._after_replay(ps, tau)
         return ps
+
+    # the pieces of a graph-replayed step; NREG overrides them
+    def _eager_graph_step(self, ps, tau):
+        return self.do_step(ps, tau)
+
+    def _capture_body(self, ps, tau):
+        self.do_step(ps, tau)
+
+    def _after_replay(self, ps, tau):
+        ps.t_curr += tau
 
     def evolve(self, t_end, callback=None):
         """Step until t_end (tupan/simulation.py:190-193); ``callback(ps)``
@@ -568,9 +578,18 @@ class NREG(Base):
     velocities recentred on the centre of mass after each sub-step.  The
     scalars the reference keeps on the particle type (E0, W, U, S) live on
     the integrator; each sub-step's length needs the last one's sums, which
-    come back to the host as in the reference."""
+    come back to the host as in the reference.
+
+    With ``graph=True`` they live in a device scalar block instead
+    (TUPAN_NREG_* slots): the sub-step lengths are computed by a one-thread
+    kernel in the same double arithmetic and read by the pair kernels from
+    device memory, so a step makes no host round trip and is replayed as a
+    HIP graph; the host reads the physical time once per step.  The same
+    bits as the host-driven step (tested).  The host attributes E0, W, U, S
+    then keep their initial values."""
 
     PROVIDED_METHODS = ["nreg", "anreg"]
+    GRAPH_METHODS = ("nreg", "anreg")
 
     def __init__(self, eta, time, ps, method, **kw):
         super(NREG, self).__init__(eta, time, ps, method, **kw)
@@ -586,6 +605,13 @@ class NREG(Base):
         self.E0 = ke + pe
         self.W = self.U = self.S = -pe
         self.is_initialized = True
+        if self.graph:
+            ps = self.ps
+            init = [0.0] * _lib.NREG_SCALARS
+            init[_lib.NREG_E0] = self.E0
+            init[_lib.NREG_W] = init[_lib.NREG_U] = init[_lib.NREG_S] = -pe
+            init[_lib.NREG_T] = init[_lib.NREG_T0] = float(ps.t_curr)
+            self._sc = torch.tensor(init, dtype=torch.float64).to(ps.device)
 
     def evolve_step(self, t_end):
         if not self.is_initialized:
@@ -622,6 +648,80 @@ class NREG(Base):
         self.anreg_step(ps, 0.5 * (h * self.S))
         self.S = 1 / (2 / self.W - 1 / self.S)
         self.anreg_step(ps, 0.5 * (h * self.S))
+
+    # ---- the step with device-resident scalars (graph=True) ----
+    def _sc_ptr(self, slot):
+        return self._sc.data_ptr() + 8 * slot
+
+    def _scalar(self, ps, op, h=0.0):
+        _lib.check(_lib.lib().tupan_nreg_scalars_dev(
+            self._sc.data_ptr(), op, float(h), _stream(ps)))
+
+    def _pair_devdt(self, ps, name):
+        k = spec.KERNELS[name]
+        f = _lib.ptr_array(ps.ptrs(k.fields))
+        _lib.check(getattr(_lib.lib(), "tupan_%s_devdt" % name)(
+            _lib.MODES.get(ps.mode, ps.mode), ps.n, f, ps.n, f,
+            self._sc_ptr(_lib.NREG_DT), _lib.ptr_array(ps.ptrs(k.outs)),
+            _stream(ps)))
+
+    def _x_dev(self, ps):
+        st = _stream(ps)
+        self._scalar(ps, _lib.NREG_OP_XDT)
+        self._pair_devdt(ps, "nreg_x")
+        _lib.call_update("divide_by", ps.n, 3, ps.ptrs(("mrx", "mry", "mrz")),
+                         ps.ptrs(("rx", "ry", "rz")),
+                         self._sc_ptr(_lib.NREG_MTOT), st)
+        _lib.call_update("sum_pairwise", ps.n, ps.field("u").data_ptr(),
+                         self._sc_ptr(_lib.NREG_SUM), st)
+        self._scalar(ps, _lib.NREG_OP_XEND)
+
+    def _v_dev(self, ps):
+        st = _stream(ps)
+        self._scalar(ps, _lib.NREG_OP_VDT)
+        self._pair_devdt(ps, "nreg_v")
+        _lib.call_update("divide_by", ps.n, 3, ps.ptrs(("mvx", "mvy", "mvz")),
+                         ps.ptrs(("vx", "vy", "vz")),
+                         self._sc_ptr(_lib.NREG_MTOT), st)
+        _lib.call_update("sum_pairwise", ps.n, ps.field("mk").data_ptr(),
+                         self._sc_ptr(_lib.NREG_SUM), st)
+        self._scalar(ps, _lib.NREG_OP_VEND)
+
+    def _anreg_dev(self, ps):
+        self._x_dev(ps)
+        self._v_dev(ps)
+        self._x_dev(ps)
+
+    def do_step_dev(self, ps, tau):
+        """do_step with every scalar on the device: no host round trip."""
+        st = _stream(ps)
+        _lib.call_update("sum_pairwise", ps.n, ps.field("mass").data_ptr(),
+                         self._sc_ptr(_lib.NREG_MTOT), st)
+        self._scalar(ps, _lib.NREG_OP_BEGIN)
+        if "anreg" in self.method:
+            self._scalar(ps, _lib.NREG_OP_H, tau / 2)
+            self._anreg_dev(ps)
+        else:
+            self._scalar(ps, _lib.NREG_OP_HS, tau)
+            self._anreg_dev(ps)
+            self._scalar(ps, _lib.NREG_OP_S)
+            self._scalar(ps, _lib.NREG_OP_HS, tau)
+            self._anreg_dev(ps)
+        _lib.call_update("nreg_step_end", ps.n, ps.tstep.data_ptr(),
+                         ps.time.data_ptr(), ps.nstep.data_ptr(),
+                         self._sc.data_ptr(), float(tau), st)
+        return ps
+
+    def _eager_graph_step(self, ps, tau):
+        ps = self.do_step_dev(ps, tau)
+        self._after_replay(ps, tau)
+        return ps
+
+    def _capture_body(self, ps, tau):
+        self.do_step_dev(ps, tau)
+
+    def _after_replay(self, ps, tau):
+        ps.t_curr = float(self._sc[_lib.NREG_T].item())
 
     def do_step(self, ps, tau):
         """nreg.py:149-163."""
