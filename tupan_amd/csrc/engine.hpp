@@ -261,55 +261,51 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
         }
       }
     }
-  } else {
-    bool dense = true;
-    if constexpr (SparsePairs<Pol>::value) {
-      if (Pol::sparse(prm)) {
-        dense = false;
-        for (int j0 = jb; j0 < je; j0 += kTile) {
-          const int nk = min(kTile, je - j0);
-          uint32_t need = 0u;  // wave-uniform: bit k = some lane needs j0 + k
-          if (prm.mask) {
-            need = prm.mask[(size_t)blockIdx.x * prm.mask_tiles + j0 / kTile];
-          } else if (nk == kTile) {
-#pragma unroll 8
-            for (int k = 0; k < kTile; ++k) {
-              const typename Pol::JRec r = jr[j0 + k];
-              bool nb = false;
+  } else if constexpr (SparsePairs<Pol>::value) {
+    // One call site of pair() (a Kepler solve: inlined once), fed per tile
+    // by a mask of the j's to visit: all of them without a cut-off, else
+    // the launcher's near mask or, without one, this wave's own test.
+    const bool sparse = Pol::sparse(prm);
+    for (int j0 = jb; j0 < je; j0 += kTile) {
+      const int nk = min(kTile, je - j0);
+      uint32_t need;  // wave-uniform: bit k = some lane needs j0 + k
+      if (!sparse) {
+        need = nk == kTile ? ~0u : (1u << nk) - 1u;
+      } else if (prm.mask) {
+        need = prm.mask[(size_t)blockIdx.x * prm.mask_tiles + j0 / kTile];
+      } else {
+        need = 0u;
+        for (int k = 0; k < nk; ++k) {
+          const typename Pol::JRec r = jr[j0 + k];
+          bool nb = false;
 #pragma unroll
-              for (int p = 0; p < P; ++p)
-                nb = nb || Pol::near(ist[p], r, prm);
-              need |= (__any(nb) ? 1u : 0u) << k;
-            }
-          } else {
-            for (int k = 0; k < nk; ++k) {
-              const typename Pol::JRec r = jr[j0 + k];
-              bool nb = false;
-#pragma unroll
-              for (int p = 0; p < P; ++p)
-                nb = nb || Pol::near(ist[p], r, prm);
-              need |= (__any(nb) ? 1u : 0u) << k;
-            }
-          }
-          while (need) {
-            const int k = __builtin_ctz(need);
-            need &= need - 1u;
-            const typename Pol::JRec r = jr[j0 + k];
-#pragma unroll
-            for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
-          }
+          for (int p = 0; p < P; ++p) nb = nb || Pol::near(ist[p], r, prm);
+          need |= (__any(nb) ? 1u : 0u) << k;
         }
       }
-    }
-    if (dense && jb < je) {
-      typename Pol::JRec r = jr[jb];
-#pragma unroll 2
-      for (int j = jb; j < je; ++j) {
-        const typename Pol::JRec rn = jr[min(j + 1, je - 1)];
+      if (!need) continue;
+      // the record of the next set bit is fetched while this one is used
+      int k = __builtin_ctz(need);
+      need &= need - 1u;
+      typename Pol::JRec r = jr[j0 + k];
+      for (;;) {
+        const int kn = need ? __builtin_ctz(need) : k;
+        const typename Pol::JRec rn = jr[j0 + kn];
 #pragma unroll
         for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
+        if (!need) break;
+        need &= need - 1u;
         r = rn;
       }
+    }
+  } else if (jb < je) {
+    typename Pol::JRec r = jr[jb];
+#pragma unroll 2
+    for (int j = jb; j < je; ++j) {
+      const typename Pol::JRec rn = jr[min(j + 1, je - 1)];
+#pragma unroll
+      for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
+      r = rn;
     }
   }
 
