@@ -8,6 +8,8 @@ the graph run's state equals the eager run's bit for bit -- the first step
 (eager on the capture stream), the second (captured and replayed) and the
 later replays alike.  The last step of each run is shorter (t_end is not a
 multiple of the step), so that step runs eagerly beside the graphs.
+Adaptive methods choose each block step eagerly (a host read) and replay
+the graph captured for that step length.
 
 NREG (nreg, anreg) replays with its scalars (E0, W, U, S, the physical
 clock) in device memory: after every step its state and physical time equal
@@ -91,12 +93,41 @@ def test_graph_replay_equals_eager(torch, method, kw, tau, compact):
     graph.integrator.check_status()
 
 
-def test_graph_rejected_for_adaptive_methods(torch):
+ADAPTIVE = [("ahermite4", 1.0 / 32, 0.01), ("sia21a.kdk", 1.0 / 32, 0.01),
+            ("asakura", 1.0 / 32, 0.01)]
+
+
+@pytest.mark.parametrize("method,eta,soft", ADAPTIVE,
+                         ids=[c[0] for c in ADAPTIVE])
+def test_adaptive_graph_replay_equals_eager(torch, method, eta, soft):
+    """Adaptive methods: the block step is chosen eagerly every step (one
+    host read of the smallest time step), and the step of that length is
+    replayed from the graph captured for it."""
     from tupan_amd.integrator import Integrator, System
-    s = System.from_host(_ics(32, 3, 0.01))
-    for method in ("ahermite4", "sia21a.kdk", "asakura"):
-        with pytest.raises(ValueError):
-            Integrator(1.0 / 64, 0.0, s, method, graph=True)
+    d = _ics(128, 32, soft)
+    eager_s, graph_s = System.from_host(d), System.from_host(d)
+    eager = Integrator(eta, 0.0, eager_s, method)
+    graph = Integrator(eta, 0.0, graph_s, method, graph=True)
+    taus = []
+    for k in range(12):
+        print("[graphs] %s step %d" % (method, k), flush=True)
+        t0 = eager_s.t_curr
+        eager.evolve_step(10.0)
+        graph.evolve_step(10.0)
+        taus.append(eager_s.t_curr - t0)
+        graph_s.diagnostics()
+        torch.cuda.synchronize()
+        a, b = eager_s.to_host(STATE), graph_s.to_host(STATE)
+        for f in STATE:
+            assert np.array_equal(a[f].view(np.uint64),
+                                  b[f].view(np.uint64)), (k, f)
+        assert graph_s.t_curr == eager_s.t_curr, k
+        assert np.array_equal(graph_s.tstep.cpu().numpy(),
+                              eager_s.tstep.cpu().numpy()), k
+    # a step length met twice has been captured and replayed
+    assert len(set(taus)) < len(taus)
+    assert graph.integrator._graphs
+    graph.integrator.check_status()
 
 
 @pytest.mark.parametrize("method,mode", [("anreg", "fp64"), ("nreg", "fp64"),

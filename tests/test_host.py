@@ -283,20 +283,3 @@ def test_integrator_rejects_unknown_methods():
         ig.Integrator(0.01, 0.0, None, "sia43s.kdk")
     assert "ahermite4" in ig.Integrator.PROVIDED_METHODS
     assert "sakura_pn" in ig.Integrator.PROVIDED_METHODS
-
-
-def test_graph_replay_only_for_fixed_step_methods():
-    """graph=True is refused, before any device work, for methods that take
-    host decisions inside a step (adaptive block-step selection)."""
-    torch = pytest.importorskip("torch")  # noqa: F841
-    from tupan_amd import integrator as ig
-    for method in ("ahermite4", "sia21a.dkd", "asakura"):
-        with pytest.raises(ValueError):
-            ig.Integrator(0.01, 0.0, None, method, graph=True)
-    fixed = set()
-    for cls in (ig.Hermite, ig.SIA, ig.Sakura, ig.SakuraPN, ig.NREG):
-        assert set(cls.GRAPH_METHODS) <= set(cls.PROVIDED_METHODS)
-        fixed |= set(cls.GRAPH_METHODS)
-    assert fixed == {"hermite2", "hermite4", "hermite6", "hermite8",
-                     "sia21s.dkd", "sia21s.kdk", "sakura", "sakura_pn",
-                     "nreg", "anreg"}

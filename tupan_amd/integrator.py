@@ -228,18 +228,10 @@ class Base(object):
 
     PROVIDED_METHODS = []
 
-    # methods whose step makes no host round trip and can be replayed as a
-    # HIP graph (fixed step lengths)
-    GRAPH_METHODS = ()
-
     def __init__(self, eta, time, ps, method, pn_order=0, clight=None,
                  mode="fp64", graph=False):
         if method not in self.PROVIDED_METHODS:
             raise ValueError("Unexpected method: {0}".format(method))
-        if graph and method not in self.GRAPH_METHODS:
-            raise ValueError("method {0} takes host decisions inside a step "
-                             "and cannot be replayed as a graph"
-                             .format(method))
         self.graph = bool(graph)
         self._graphs = {}
         self._seen = set()
@@ -286,9 +278,24 @@ class Base(object):
     def evolve_step(self, t_end):
         tau = self.get_base_tstep(t_end)
         if self.graph:
+            # the step length is a host decision (adaptive methods read the
+            # smallest time step); the step of that length is a graph
+            tau = self.decide_tstep(self.ps, tau)
             self.ps = self._graph_step(self.ps, tau)
         else:
             self.ps = self.do_step(self.ps, tau)
+
+    # A step is decide_tstep (host side: the block step of the adaptive
+    # methods, tau itself otherwise) followed by fixed_step (device work
+    # only, replayable).
+    def decide_tstep(self, ps, tau):
+        return tau
+
+    def fixed_step(self, ps, tau):
+        raise NotImplementedError
+
+    def do_step(self, ps, tau):
+        return self.fixed_step(ps, self.decide_tstep(ps, tau))
 
     def _graph_step(self, ps, tau):
         """One step of length ``tau`` as a HIP graph replay.  The first step
@@ -322,10 +329,10 @@ class Base(object):
 
     # the pieces of a graph-replayed step; NREG overrides them
     def _eager_graph_step(self, ps, tau):
-        return self.do_step(ps, tau)
+        return self.fixed_step(ps, tau)
 
     def _capture_body(self, ps, tau):
-        self.do_step(ps, tau)
+        self.fixed_step(ps, tau)
 
     def _after_replay(self, ps, tau):
         ps.t_curr += tau
@@ -358,7 +365,6 @@ class Hermite(Base):
 
     PROVIDED_METHODS = ["hermite2", "ahermite2", "hermite4", "ahermite4",
                         "hermite6", "ahermite6", "hermite8", "ahermite8"]
-    GRAPH_METHODS = ("hermite2", "hermite4", "hermite6", "hermite8")
 
     def __init__(self, eta, time, ps, method, **kw):
         super(Hermite, self).__init__(eta, time, ps, method, **kw)
@@ -396,9 +402,12 @@ class Hermite(Base):
                              float(tau), st)
         return ps
 
-    def do_step(self, ps, tau):
+    def decide_tstep(self, ps, tau):
         if "ahermite" in self.method:
             tau = self.get_hermite_tstep(ps, self.eta, tau)
+        return tau
+
+    def fixed_step(self, ps, tau):
         ps = self.epec(2, ps, tau)
         self._advance_clock(ps, tau)
         return ps
@@ -410,7 +419,6 @@ class SIA(Base):
 
     PROVIDED_METHODS = ["sia21s.dkd", "sia21a.dkd", "sia21s.kdk",
                         "sia21a.kdk"]
-    GRAPH_METHODS = ("sia21s.dkd", "sia21s.kdk")
 
     def __init__(self, eta, time, ps, method, **kw):
         super(SIA, self).__init__(eta, time, ps, method, **kw)
@@ -461,13 +469,16 @@ class SIA(Base):
         self.kick(ps, 1.0 * tau)
         self.drift(ps, 0.5 * tau)
 
-    def do_step(self, ps, tau):
+    def decide_tstep(self, ps, tau):
         if ps.n <= 2:
             raise NotImplementedError("n <= 2 needs the few-body solver "
                                       "(tupan/integrator/fewbody.py)")
         if "a." in self.method:
             ps.set_tstep(self.eta)
             tau = self.get_min_block_tstep(ps, tau)
+        return tau
+
+    def fixed_step(self, ps, tau):
         # shared steps: every particle is slow, and SIA21.bridge_sf evolves
         # the slow set twice by d[0]*tau = tau/2 (sia.py:341-353)
         ev = self.kdk if "kdk" in self.method else self.dkd
@@ -481,7 +492,6 @@ class Sakura(Base):
     """Sakura pairwise Kepler splitting (sakura.py:22-162)."""
 
     PROVIDED_METHODS = ["sakura", "asakura"]
-    GRAPH_METHODS = ("sakura",)
 
     def sakura_step(self, ps, tau):
         r = ps.ptrs(("rx", "ry", "rz"))
@@ -510,9 +520,12 @@ class Sakura(Base):
         ps.tstep.fill_(float(dt_sakura))
         return self.get_min_block_tstep(ps, tau)
 
-    def do_step(self, ps, tau):
+    def decide_tstep(self, ps, tau):
         if "asakura" in self.method:
             tau = self.get_sakura_tstep(ps, self.eta, tau)
+        return tau
+
+    def fixed_step(self, ps, tau):
         ps = self.sakura_step(ps, tau)
         self._advance_clock(ps, tau)
         return ps
@@ -531,7 +544,6 @@ class SakuraPN(SIA):
     Shared fixed steps tau = min(t_end - t, eta)."""
 
     PROVIDED_METHODS = ["sakura_pn"]
-    GRAPH_METHODS = ("sakura_pn",)
 
     def __init__(self, eta, time, ps, method, rcut=1e-3, **kw):
         super(SIA, self).__init__(eta, time, ps, method, **kw)
@@ -563,7 +575,10 @@ class SakuraPN(SIA):
                 ps.ptrs(("pn_mvx", "pn_mvy", "pn_mvz")), float(tau), 0, None,
                 None, st)
 
-    def do_step(self, ps, tau):
+    def decide_tstep(self, ps, tau):
+        return tau
+
+    def fixed_step(self, ps, tau):
         self.kick(ps, 0.5 * tau)
         self.drift(ps, tau)
         self.kick(ps, 0.5 * tau)
@@ -589,7 +604,6 @@ class NREG(Base):
     then keep their initial values."""
 
     PROVIDED_METHODS = ["nreg", "anreg"]
-    GRAPH_METHODS = ("nreg", "anreg")
 
     def __init__(self, eta, time, ps, method, **kw):
         super(NREG, self).__init__(eta, time, ps, method, **kw)
