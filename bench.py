@@ -124,14 +124,20 @@ def time_steps(fn, steps, warmup, torch):
     torch.cuda.synchronize()
     _lib.take_kernel_times()  # drop warm-up launches
     _lib.set_timing(True)
+    ev = [(torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for a, b in ev:
+        a.record()
         fn()
+        b.record()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     _lib.set_timing(False)
     k_ms, k_n = _lib.take_kernel_times()
+    # per-step device times (events on the launch stream around each step)
+    time_steps.step_ms = sorted(a.elapsed_time(b) for a, b in ev)
     return elapsed, k_ms / max(k_n, 1), k_n
 
 
@@ -790,6 +796,7 @@ def main():
 
     print("[bench] timing %s x%d" % (args.mode, args.steps), file=sys.stderr, flush=True)
     elapsed, k_ms, k_n = time_steps(step, args.steps, args.warmup, torch)
+    step_ms = time_steps.step_ms
     pairs = float(n) * n
     value = pairs * args.steps / elapsed / 1e9
     ms_per_step = 1e3 * elapsed / args.steps
@@ -815,6 +822,10 @@ def main():
             "n": n, "mode": args.mode,
             "interactions_per_step": pairs,
         },
+        "step_ms": {"median": float(np.median(step_ms)),
+                    "min": float(step_ms[0]), "max": float(step_ms[-1]),
+                    "note": "device time of each timed step (HIP events on "
+                            "the launch stream)"},
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
