@@ -11,6 +11,14 @@
 //                    accumulate straight into A64
 //   load_i, pair, fold, combine, finish
 //
+// and optionally
+//
+//   kSparse, sparse, near   pairs that are mostly no-ops (Sakura with a
+//                           cut-off): tiles are visited through a mask of
+//                           the j's some lane needs (near_mask_kernel)
+//   prepare                 parameters resolved from device memory when the
+//                           kernel starts (NREG's dt in device-resident runs)
+//
 // Execution model (CDNA4, wave64):
 //   * a wave owns 64*P consecutive i-particles, P per lane, in VGPRs;
 //   * all 64 lanes walk the same j sequence, so a j record is wave-uniform:
