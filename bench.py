@@ -312,11 +312,11 @@ def bench_ensemble(args, torch):
 
 def bench_sizes(args, torch):
     """The single-evaluation kernels at N in {16384, 65536, 262144}: Plummer
-    seed 1, Salpeter masses [0.1, 100], eps = 4/N.  Acc, AccJerk and
-    SnapCrackle (its a, j inputs from the FP64 AccJerk) in FP64 and FP32C:
-    interactions/s from the live engine timing, and per-particle relative
-    errors |x - x64| / |x64| (max, RMS) of every output vector against the
-    FP64 kernel."""
+    seed 1, Salpeter masses [0.1, 100], eps = 4/N.  Phi, Acc, AccJerk and
+    SnapCrackle (its a, j inputs from the FP64 AccJerk) in FP64 and FP32C,
+    Tstep (eta = 0.02) in FP64 (its only mode): interactions/s from the live
+    engine timing, and per-particle relative errors |x - x64| / |x64| (max,
+    RMS) of every FP32C output against the FP64 kernel."""
     from tupan_amd import device, spec
     rows = []
     for n in (16384, 65536, 262144):
@@ -326,21 +326,33 @@ def bench_sizes(args, torch):
         aj = device.compute("acc_jerk", d, d, mode="fp64")
         for k, f in enumerate(("ax", "ay", "az", "jx", "jy", "jz")):
             d[f] = aj[k].clone()
-        for name in ("acc", "acc_jerk", "snap_crackle"):
+        for name in ("phi", "acc", "acc_jerk", "snap_crackle", "tstep"):
             res = {}
-            for mode in ("fp64", "fp32c"):
+            modes = ("fp64",) if name == "tstep" else ("fp64", "fp32c")
+            extra = (0.02,) if name == "tstep" else ()
+            for mode in modes:
                 out = tuple(torch.empty(n, dtype=torch.float64, device="cuda")
                             for _ in spec.KERNELS[name].outs)
-                step = lambda: device.compute(name, d, d, mode=mode, out=out)
+                step = lambda: device.compute(name, d, d, *extra, mode=mode,
+                                              out=out)
                 reps = max(2, min(20, int(2e11 / (n * n))))
                 _, k_ms, _ = time_steps(step, reps, 1, torch)
                 res[mode] = (out, k_ms)
             pairs = float(n) * n
-            row = {"n": n, "kernel": name}
-            for mode in ("fp64", "fp32c"):
+            row = {"n": n, "kernel": name,
+                   "flops_per_interaction": spec.KERNELS[name].flops}
+            for mode in modes:
                 row[mode + "_Gint_s"] = pairs / (res[mode][1] * 1e-3) / 1e9
                 row[mode + "_kernel_ms"] = res[mode][1]
+            if "fp32c" not in res:
+                rows.append(row)
+                print(json.dumps(row), file=sys.stderr, flush=True)
+                continue
             got, ref = res["fp32c"][0], res["fp64"][0]
+            if name == "phi":
+                e = (got[0] - ref[0]).abs() / ref[0].abs()
+                row["phi_max"] = float(e.max())
+                row["phi_rms"] = float(e.pow(2).mean().sqrt())
             for g in range(len(got) // 3):
                 G = torch.stack(got[3 * g:3 * g + 3])
                 R = torch.stack(ref[3 * g:3 * g + 3])
@@ -360,7 +372,7 @@ def bench_sizes(args, torch):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "Plummer seed 1, Salpeter [0.1,100], eps=4/N;"
-                               " Acc, AccJerk, SnapCrackle"},
+                               " Phi, Acc, AccJerk, SnapCrackle, Tstep"},
         "rows": rows,
     }
     print(json.dumps(result))
