@@ -199,6 +199,11 @@ def take_kernel_times():
 
 
 def ptr_array(ptrs):
+    """ctypes void* array of the pointers (a cached one when ``ptrs``
+    carries it, see integrator.System.ptrs)."""
+    cached = getattr(ptrs, "carr", None)
+    if cached is not None:
+        return cached
     arr = (ctypes.c_void_p * max(1, len(ptrs)))()
     for k, p in enumerate(ptrs):
         arr[k] = p

@@ -45,6 +45,10 @@ PN_FIELDS = ("wx", "wy", "wz", "pn_ke", "pn_mrx", "pn_mry", "pn_mrz",
              "pn_mvx", "pn_mvy", "pn_mvz", "pn_amx", "pn_amy", "pn_amz")
 
 
+class _Ptrs(tuple):
+    """A tuple of device pointers with its ctypes array attached."""
+
+
 class System(object):
     """SoA particle set in HBM: ``ps.<field>`` is a contiguous float64
     device tensor of length n.  Fields beyond the base set are created on
@@ -53,6 +57,7 @@ class System(object):
 
     def __init__(self, n, device="cuda"):
         object.__setattr__(self, "_f", {})
+        self._ptrs = {}
         self.n = int(n)
         self.device = torch.device(device)
         self.t_curr = 0.0
@@ -102,8 +107,22 @@ class System(object):
     def _pair(self, name, *args, mode=None, jps=None, status=None):
         k = spec.KERNELS[name]
         out = tuple(self.field(o) for o in k.outs)
-        device.compute(name, self, self if jps is None else jps, *args,
-                       mode=mode or self.mode, out=out, status=status)
+        if jps is not None:
+            device.compute(name, self, jps, *args, mode=mode or self.mode,
+                           out=out, status=status)
+            return out
+        # the set with itself: its own fields are contiguous float64 device
+        # tensors of length n by construction, so the checks of
+        # device.compute are skipped and the cached pointer arrays used
+        sptr = None
+        if status is not None:
+            if status.dtype != torch.int32 or not status.is_cuda:
+                raise ValueError("status must be an int32 device tensor")
+            sptr = status.data_ptr()
+        f = self.ptrs(k.fields)
+        _lib.call_dev(name, mode or self.mode, self.n, f, self.n, f,
+                      self.ptrs(k.outs), args, stream=_stream(self),
+                      status_ptr=sptr)
         return out
 
     def set_phi(self, mode=None):
@@ -215,7 +234,16 @@ class System(object):
         return float(self.tstep.abs().min())
 
     def ptrs(self, names):
-        return [self.field(f).data_ptr() for f in names]
+        """The device pointers of the named fields, as a tuple that also
+        carries its ctypes array (``carr``).  Fields are created once and
+        never replaced, so the result is cached per name tuple."""
+        key = tuple(names)
+        p = self._ptrs.get(key)
+        if p is None:
+            p = _Ptrs(self.field(f).data_ptr() for f in key)
+            p.carr = _lib.ptr_array(list(p))
+            self._ptrs[key] = p
+        return p
 
 
 def _stream(ps):
@@ -373,6 +401,8 @@ class Hermite(Base):
         self.names = ("rx", "ry", "rz", "vx", "vy", "vz") + DERIV[:nd - 6]
         self.shadow = {f: torch.empty(ps.n, dtype=torch.float64,
                                       device=ps.device) for f in self.names}
+        self._x0 = _lib.ptr_array([self.shadow[f].data_ptr()
+                                   for f in self.names])
 
     def _derivs(self, ps):
         if self.order == 2:
@@ -392,7 +422,7 @@ class Hermite(Base):
         # whole snapshot to the shadow set before advancing ps.
         self._derivs(ps)
         x = ps.ptrs(self.names)
-        x0 = [self.shadow[f].data_ptr() for f in self.names]
+        x0 = self._x0
         st = _stream(ps)
         _lib.call_update("hermite_predict", self.order, ps.n, x, x0,
                          float(tau), st)
