@@ -189,12 +189,23 @@ __device__ __forceinline__ void prepare_params(typename Pol::Params &prm) {
 // ------------------------------------------------------------- engine
 // FP32C policies (kTwoLevel) also receive the caller's FP64 arrays and the
 // j index: a pair they classify as "near" is re-evaluated from those in FP64.
-template <class Pol, int P>
+// Direct (FP64 policies, small nj): no packed records -- each wave-uniform
+// j record is built from the caller's SoA arrays with scalar loads where it
+// is used, which saves the packing launch when that launch would cost more
+// than the pair loop gains from one-record loads.
+template <class Pol, int P, bool Direct = false>
 __global__ __launch_bounds__(kBlock) void pair_engine(
     Soa is, Soa js, int ni, const typename Pol::JRec *__restrict__ jr, int nj,
     int jchunk, int nrows, double *__restrict__ part, Out out,
     typename Pol::Params prm, const double *__restrict__ aux) {
+  static_assert(!(Direct && Pol::kTwoLevel), "FP32C records need packing");
   prepare_params<Pol>(prm);
+  auto rec = [&](int j) -> typename Pol::JRec {
+    if constexpr (Direct)
+      return Pol::pack(js, j, 0.0);
+    else
+      return jr[j];
+  };
   const int lane = threadIdx.x & 63;
   // wave-uniform by construction; readfirstlane lets the compiler keep the
   // j range, and so the j-record loads, scalar
@@ -284,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
       } else {
         need = 0u;
         for (int k = 0; k < nk; ++k) {
-          const typename Pol::JRec r = jr[j0 + k];
+          const typename Pol::JRec r = rec(j0 + k);
           bool nb = false;
 #pragma unroll
           for (int p = 0; p < P; ++p) nb = nb || Pol::near(ist[p], r, prm);
@@ -295,10 +306,10 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
       // the record of the next set bit is fetched while this one is used
       int k = __builtin_ctz(need);
       need &= need - 1u;
-      typename Pol::JRec r = jr[j0 + k];
+      typename Pol::JRec r = rec(j0 + k);
       for (;;) {
         const int kn = need ? __builtin_ctz(need) : k;
-        const typename Pol::JRec rn = jr[j0 + kn];
+        const typename Pol::JRec rn = rec(j0 + kn);
 #pragma unroll
         for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
         if (!need) break;
@@ -307,10 +318,10 @@ __global__ __launch_bounds__(kBlock) void pair_engine(
       }
     }
   } else if (jb < je) {
-    typename Pol::JRec r = jr[jb];
+    typename Pol::JRec r = rec(jb);
 #pragma unroll 2
     for (int j = jb; j < je; ++j) {
-      const typename Pol::JRec rn = jr[min(j + 1, je - 1)];
+      const typename Pol::JRec rn = rec(min(j + 1, je - 1));
 #pragma unroll
       for (int p = 0; p < P; ++p) Pol::pair(ist[p], r, acc[p], prm);
       r = rn;

@@ -57,11 +57,24 @@ int run_ieee_tstep(size_t ni, const double *const *ip, size_t nj,
 double near_field_factor();
 int sakura_budget();
 
+// FP64 policies read their j records straight from the caller's arrays
+// (pair_engine's Direct variant) up to this many j's: below it the packing
+// launch costs more than one-record loads save.
+constexpr int kDirectMaxJ = 8192;
+
 template <class Pol, int P>
 static void launch_p(dim3 grid, hipStream_t st, const Soa &is, const Soa &js,
                      int ni, const typename Pol::JRec *jr, int nj, int jchunk,
                      int nrows, double *part, const Out &out,
                      const typename Pol::Params &prm, const double *aux) {
+  if constexpr (!Pol::kTwoLevel) {
+    if (!jr) {
+      hipLaunchKernelGGL((pair_engine<Pol, P, true>), grid, dim3(kBlock), 0,
+                         st, is, js, ni, jr, nj, jchunk, nrows, part, out,
+                         prm, aux);
+      return;
+    }
+  }
   hipLaunchKernelGGL((pair_engine<Pol, P>), grid, dim3(kBlock), 0, st, is,
                      js, ni, jr, nj, jchunk, nrows, part, out, prm, aux);
 }
@@ -123,7 +136,9 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   const size_t ctx_bytes =
       Pol::kTwoLevel ? sizeof(double) * (2 + 4 * ntiles + 2 * kCtxBlocks)
                      : 0;
-  if (n_j > 0) {
+  // FP64 policies at small nj build their records in the pair kernel
+  const bool direct = !Pol::kTwoLevel && n_j <= kDirectMaxJ;
+  if (n_j > 0 && !direct) {
     rc = scratch_alloc(kSlotRecords, (void **)&jr, rec_bytes + ctx_bytes, st);
     if (rc) return rc;
     if (Pol::kTwoLevel)
@@ -151,8 +166,9 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
                            dim3((unsigned)((ntiles + 255) / 256)), dim3(256),
                            0, st, js.f[VX], js.f[VY], js.f[VZ], n_j, ctx + 2);
     }
-    hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
-                       dim3(256), 0, st, js, n_j, jr, (const double *)ctx);
+    if (!direct)
+      hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),
+                         dim3(256), 0, st, js, n_j, jr, (const double *)ctx);
   }
   // sparse policies: the near mask, when it fits (P = 1 only)
   typename Pol::Params prm2 = prm;
