@@ -59,8 +59,9 @@ int sakura_budget();
 
 // FP64 policies read their j records straight from the caller's arrays
 // (pair_engine's Direct variant) up to this many j's: below it the packing
-// launch costs more than one-record loads save.
-constexpr int kDirectMaxJ = 8192;
+// launch costs more than one-record loads save (measured: the Direct pair
+// loop is ~13% slower at nj = 16384 and ~40% at 65536 for AccJerk).
+constexpr int kDirectMaxJ = 4096;
 
 template <class Pol, int P>
 static void launch_p(dim3 grid, hipStream_t st, const Soa &is, const Soa &js,
