@@ -10,7 +10,9 @@
 * a call that would have to grow a buffer while its stream is being
   captured fails with a message, before launching anything, and the
   capture ends cleanly; after the same call has run once eagerly on that
-  stream it captures, and the replays match the eager result.
+  stream it captures, and the replays match the eager result;
+* tupan_release_scratch frees the cache, and later calls rebuild it and
+  give the same bits.
 """
 import threading
 
@@ -143,3 +145,17 @@ def test_capture_needs_an_eager_call_first(torch):
         g.replay()
         torch.cuda.synchronize()
         assert all(np.array_equal(x, y) for x, y in zip(_bits(out), want))
+
+
+def test_release_then_reuse(torch):
+    from tupan_amd import _lib
+    d = _set(torch, 4000, 9)
+    before = [_bits(_acc_jerk(torch, d, m)) for m in ("fp64", "fp32c",
+                                                       "fp64_ieee")]
+    torch.cuda.synchronize()
+    _lib.release_scratch()
+    for _ in range(2):
+        after = [_bits(_acc_jerk(torch, d, m)) for m in ("fp64", "fp32c",
+                                                          "fp64_ieee")]
+        for x, y in zip(before, after):
+            assert all(np.array_equal(a, b) for a, b in zip(x, y))

@@ -186,6 +186,13 @@ def set_sakura_budget(solves):
     check(lib().tupan_set_sakura_budget(int(solves)))
 
 
+def release_scratch():
+    """Free the per-stream scratch buffers of the device entry points
+    (tupan_release_scratch): graphs captured before must not be replayed
+    afterwards; later calls allocate afresh."""
+    check(lib().tupan_release_scratch())
+
+
 def set_timing(on):
     check(lib().tupan_set_timing(1 if on else 0))
 
