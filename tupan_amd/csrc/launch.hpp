@@ -19,7 +19,7 @@ struct LaunchShape {
   int splits;  // j-range splits
 };
 LaunchShape choose_shape(int kernel_id, int mode, size_t ni, size_t nj,
-                         int pref_ipl, int max_ipl);
+                         int pref_ipl, int max_ipl, int min_chunk);
 int device_alloc(void **p, size_t bytes, hipStream_t st);
 void device_free(void *p, hipStream_t st);
 // Reusable per-(device, stream, slot) device buffers for the asynchronous
@@ -111,16 +111,20 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     out.f[k] = op[k];
   }
   const int n_i = (int)ni, n_j = (int)nj;
+  // j chunks are whole tiles for the tiled loops (FP32C, sparse policies);
+  // the plain FP64 loop takes chunks of 8 records, which tiny systems use
+  constexpr int kChunk =
+      (Pol::kTwoLevel || SparsePairs<Pol>::value) ? kTile : 8;
   LaunchShape sh = choose_shape(kernel_id, mode, ni, nj, Pol::kPrefIpl,
-                                Pol::kMaxIpl);
+                                Pol::kMaxIpl, kChunk);
   const int P = sh.ipl;
-  // S j chunks of jchunk records (a multiple of kTile), dealt kWaves per
+  // S j chunks of jchunk records (a multiple of kChunk), dealt kWaves per
   // block: SB block rows (grid: i-waves x SB)
   int S = sh.splits < kWaves ? kWaves : sh.splits;
   S = (S + kWaves - 1) / kWaves * kWaves;
   int jchunk = (n_j + S - 1) / S;
-  jchunk = (jchunk + kTile - 1) / kTile * kTile;
-  if (jchunk < kTile) jchunk = kTile;
+  jchunk = (jchunk + kChunk - 1) / kChunk * kChunk;
+  if (jchunk < kChunk) jchunk = kChunk;
   const int used = (n_j + jchunk - 1) / jchunk;  // chunks holding records
   const int SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
   const int waves = (n_i + 64 * P - 1) / (64 * P);  // i-waves

@@ -64,7 +64,7 @@ static int simd_count() {
 // lifetime (measured best on MI355X for N = 16K..64K), never splitting j
 // below 128 records per wave.
 LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
-                         int pref_ipl, int max_ipl) {
+                         int pref_ipl, int max_ipl, int min_chunk) {
   LaunchShape o{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -73,10 +73,11 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
       o = g_override[kid][mode];
   }
   const long target = 64L * simd_count();
-  // j chunks of at least 128 records, or of 32 (one tile) while that keeps
-  // the split count at 64 or below: small systems need the extra waves.
+  // j chunks of at least 128 records, or of min_chunk (a tile, or 8 for the
+  // plain FP64 loop) while that keeps the split count at 64 or below: small
+  // systems need the extra waves.
   long max_split = nj >= 256 ? (long)(nj / 128) : 1;
-  const long fine = std::min<long>((long)(nj / 32), 64);
+  const long fine = std::min<long>((long)(nj / (size_t)min_chunk), 64);
   if (fine > max_split) max_split = fine;
   int P = o.ipl > 0 ? o.ipl : pref_ipl;
   if (P > max_ipl) P = max_ipl;
