@@ -268,6 +268,20 @@ def bench_ensemble(args, torch):
     elapsed = time.perf_counter() - t0
     ens_rate = (p1 - p0) / elapsed
     ens.close()
+    # the same steps with the AccJerk sums in FP32C arithmetic
+    print("[bench] fp32c ensemble", file=sys.stderr, flush=True)
+    e32 = ensemble.Ensemble(sizes, 0.02, 1.0, mode="fp32c")
+    best32, _ = e32.autotune(fields, trial_steps=4,
+                             candidates=ENS_CANDIDATES, use_cache=False)
+    e32.set_state(fields)
+    e32.step(args.warmup)
+    _, q0, _ = e32.poll()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e32.step(args.steps)
+    _, q1, _ = e32.poll()
+    el32 = time.perf_counter() - t0
+    e32.close()
     # baseline: looping the single-system path per cluster
     sample = list(range(0, len(sizes), 16))
     off = np.concatenate([[0], np.cumsum(sizes)])
@@ -302,6 +316,10 @@ def bench_ensemble(args, torch):
                    "particles": int(sizes.sum()),
                    "launch": list(best)},
         "autotune": {"%d,%d,%d" % k: v / 1e9 for k, v in rates.items()},
+        "fp32c": {"value": (q1 - q0) / el32 / 1e9,
+                  "unit": "Ginteractions/s",
+                  "ms_per_step": 1e3 * el32 / args.steps,
+                  "launch": list(best32)},
         "per_cluster_loop": {"value": loop_rate / 1e9,
                              "unit": "Ginteractions/s",
                              "clusters": len(sample),

@@ -291,8 +291,12 @@ int tupan_phi_adjoint_dev(int mode, size_t ni, const double *const *ip,
  * sums of all clusters into one persistent kernel per phase.  One ensemble
  * step (Tstep, block-step choice, AccJerk + predict, 2 x (AccJerk +
  * correct), clock) is captured once into a HIP graph and replayed.
- * FP64 arithmetic throughout.  Device memory is allocated at create and at
- * the first step after set_launch; steps allocate nothing. */
+ * FP64 arithmetic by default; tupan_ensemble_set_mode(TUPAN_MODE_FP32C)
+ * evaluates the AccJerk sums in the FP32C arithmetic of the single-system
+ * kernels (near-field factor and velocity tile origins per cluster), the
+ * Tstep sums and the updates stay FP64.  Device memory is allocated at
+ * create and at the first step after set_launch or set_mode; steps
+ * allocate nothing. */
 typedef struct tupan_ensemble tupan_ensemble;
 /* sizes[c] = particles of cluster c; clusters are stored back to back. */
 int tupan_ensemble_create(tupan_ensemble **out, int nclusters,
@@ -302,6 +306,17 @@ int tupan_ensemble_destroy(tupan_ensemble *e);
  * item, resident waves per SIMD of the persistent kernels. */
 int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
                               int waves_per_simd);
+/* mode = TUPAN_MODE_FP64 (default) or TUPAN_MODE_FP32C (AccJerk only; one
+ * i-particle per lane, jchunk a multiple of 32). */
+int tupan_ensemble_set_mode(tupan_ensemble *e, int mode);
+/* Storage order of each cluster's particles on the device, applied at the
+ * next set_state (get_state returns input order either way):
+ * TUPAN_ENS_ORDER_MORTON (default) sorts them by the Morton code of their
+ * positions, so that neighbours share waves and j tiles;
+ * TUPAN_ENS_ORDER_INPUT keeps the input order. */
+#define TUPAN_ENS_ORDER_INPUT 0
+#define TUPAN_ENS_ORDER_MORTON 1
+int tupan_ensemble_set_order(tupan_ensemble *e, int order);
 /* fields = host {m, rx, ry, rz, e2, vx, vy, vz}, each of the total length;
  * t0 = per-cluster start times (NULL = 0).  Synchronous. */
 int tupan_ensemble_set_state(tupan_ensemble *e, const double *const *fields,

@@ -8,6 +8,15 @@ on the C kernels, for the smallest clusters.  States agree to 1e-11 of each
 component's scale (the pair sums are split differently, so only rounding
 differs).  The captured-graph replay and the launched-kernel path give
 identical bits, and the autotuner's launch shapes agree to 1e-11.
+
+FP32C mode (AccJerk sums in FP32C arithmetic): every cluster takes the
+same steps as in FP64 and ends within 1e-7 of each component's scale; the
+graph replay is bitwise the launched kernels; j-chunk lengths change only
+the FP64 grouping of the tile sums (1e-11); bad modes and j chunks that are
+not whole tiles are refused.  The device storage order (Morton or input
+order) changes only the order of the pair sums (1e-11 in FP64; in FP32C
+the near field moves with the tiles, so 1e-7); states come back in input
+order.
 """
 import numpy as np
 import pytest
@@ -34,9 +43,10 @@ def setup():
     return sizes, fields
 
 
-def _run(sizes, fields, graph=True, launch=None):
+def _run(sizes, fields, graph=True, launch=None, mode="fp64",
+         order="morton"):
     from tupan_amd import ensemble
-    ens = ensemble.Ensemble(sizes, ETA, T_END)
+    ens = ensemble.Ensemble(sizes, ETA, T_END, mode=mode, order=order)
     if launch:
         ens.set_launch(*launch)
     ens.set_state(fields)
@@ -119,3 +129,55 @@ def test_launch_shapes_and_autotuner(setup, tmp_path, monkeypatch):
     ens.close()
     assert np.array_equal(ns, ref[2])
     assert _scale_err(st, ref[0]) < 1e-11
+
+
+def test_fp32c_follows_fp64(setup):
+    sizes, fields = setup
+    ref = _run(sizes, fields)
+    got = _run(sizes, fields, mode="fp32c")
+    assert np.array_equal(got[2], ref[2])
+    err = _scale_err(got[0], ref[0])
+    print("[ens fp32c] scale error %.2e" % err, flush=True)
+    assert 0.0 < err < 1e-7
+
+
+def test_fp32c_graph_and_chunks(setup):
+    sizes, fields = setup
+    a = _run(sizes, fields, graph=True, mode="fp32c")
+    b = _run(sizes, fields, graph=False, mode="fp32c")
+    for f in STATE:
+        assert np.array_equal(a[0][f], b[0][f])
+    assert np.array_equal(a[2], b[2])
+    for launch in ((2, 64, 4), (1, 1024, 16)):
+        got = _run(sizes, fields, launch=launch, mode="fp32c")
+        assert np.array_equal(got[2], a[2])
+        assert _scale_err(got[0], a[0]) < 1e-11
+
+
+def test_fp32c_arguments(setup):
+    from tupan_amd import _lib, ensemble
+    sizes, _ = setup
+    ens = ensemble.Ensemble(sizes, ETA, T_END)
+    with pytest.raises(ValueError):
+        ens.set_mode("fp64_ieee")
+    with pytest.raises(_lib.TupanError):
+        _lib.check(_lib.lib().tupan_ensemble_set_mode(ens._h,
+                                                      _lib.MODE_FP64_IEEE))
+    ens.set_launch(1, 48, 8)
+    with pytest.raises(_lib.TupanError, match="multiple"):
+        ens.set_mode("fp32c")
+    ens.set_launch(1, 64, 8)
+    ens.set_mode("fp32c")
+    with pytest.raises(_lib.TupanError, match="multiple"):
+        ens.set_launch(1, 80, 8)
+    ens.close()
+
+
+@pytest.mark.parametrize("mode,tol", [("fp64", 1e-11), ("fp32c", 1e-7)])
+def test_storage_order(setup, mode, tol):
+    sizes, fields = setup
+    a = _run(sizes, fields, mode=mode)
+    b = _run(sizes, fields, mode=mode, order="input")
+    assert np.array_equal(a[2], b[2])
+    err = _scale_err(a[0], b[0])
+    assert 0.0 < err < tol, err

@@ -147,6 +147,8 @@ def lib():
                                             _D]
         L.tupan_ensemble_destroy.argtypes = [_P]
         L.tupan_ensemble_set_launch.argtypes = [_P, _I, _I, _I]
+        L.tupan_ensemble_set_mode.argtypes = [_P, _I]
+        L.tupan_ensemble_set_order.argtypes = [_P, _I]
         L.tupan_ensemble_set_state.argtypes = [_P, _PP, _PD]
         L.tupan_ensemble_get_state.argtypes = [
             _P, _PP, _PD, ctypes.POINTER(ctypes.c_longlong)]

@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "accjerk.hpp"
 #include "hermite.hpp"
 #include "launch.hpp"
 
@@ -243,6 +244,139 @@ __global__ __launch_bounds__(kThreads) void ens_pack(int ntot, Fields s,
   }
 }
 
+// ---------------------------------------------------------------- FP32C
+// AccJerk in the FP32C arithmetic of the single-system kernels (f32c.hpp,
+// accjerk.hpp): FP64 separations rounded once to FP32, FP32 tile sums folded
+// into FP64, velocity offsets to per-tile FP64 origins, and near pairs (below
+// the j record's threshold) re-evaluated in FP64 from the slab.  Tiles are
+// numbered within each cluster (tbase: the cluster's first tile), so a work
+// item's j range -- a multiple of the tile length from the cluster's start --
+// covers whole tiles.  The near-field threshold uses each cluster's own
+// factor and largest e2 (ClusterCtx, set with the state).
+using AJ32 = AccJerkF32c;
+
+struct ClusterCtx {
+  double factor, e2max;
+  int tbase, pad;
+};
+
+// Velocity origin of every tile of the active clusters: the midpoint of the
+// tile's velocity range (tile_origin_kernel's rule), one thread per tile.
+__global__ __launch_bounds__(kThreads) void ens_origins(
+    int ntiles, const int *__restrict__ tcid, const ClusterInfo *__restrict__ cl,
+    const ClusterCtx *__restrict__ cx, const int *__restrict__ active,
+    Fields s, double *__restrict__ orig) {
+  for (int g = blockIdx.x * kThreads + threadIdx.x; g < ntiles;
+       g += gridDim.x * kThreads) {
+    const int c = tcid[g];
+    if (!active[c]) continue;
+    const ClusterInfo ci = cl[c];
+    const int j0 = ci.off + (g - cx[c].tbase) * kTile;
+    const int je = min(ci.off + ci.n, j0 + kTile);
+    for (int k = 0; k < 3; ++k) {
+      const double *v = s.f[FVX + k];
+      double lo = v[j0], hi = lo;
+      for (int j = j0 + 1; j < je; ++j) {
+        lo = fmin(lo, v[j]);
+        hi = fmax(hi, v[j]);
+      }
+      orig[4 * g + k] = 0.5 * (lo + hi);
+    }
+  }
+}
+
+// FP32C j records (AccJerkF32c::pack with the cluster's tile origin and
+// near-field context).
+__global__ __launch_bounds__(kThreads) void ens_pack32(
+    int ntot, Fields s, const int *__restrict__ cid,
+    const ClusterInfo *__restrict__ cl, const ClusterCtx *__restrict__ cx,
+    const int *__restrict__ active, const double *__restrict__ orig,
+    AJ32::JRec *__restrict__ rec) {
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < ntot;
+       i += gridDim.x * kThreads) {
+    const int c = cid[i];
+    if (!active[c]) continue;
+    const ClusterCtx x = cx[c];
+    const double *o = orig + 4 * (x.tbase + (i - cl[c].off) / kTile);
+    const PackCtx pc{x.factor, x.e2max, nullptr};
+    rec[i] = {s.f[FX][i],
+              s.f[FY][i],
+              s.f[FZ][i],
+              dsub32(s.f[FVX][i], o[0]),
+              dsub32(s.f[FVY][i], o[1]),
+              dsub32(s.f[FVZ][i], o[2]),
+              (float)s.f[FM][i],
+              (float)s.f[FE2][i],
+              near_threshold(s.f[FM][i], s.f[FE2][i], pc)};
+  }
+}
+
+// Persistent FP32C AccJerk over the work items (ens_pairs' scheme, one
+// i-particle per lane), writing the same FP64 chunk partials.
+__global__ __launch_bounds__(kThreads) void ens_pairs32(
+    const AJ32::JRec *__restrict__ rec, const Item *__restrict__ items,
+    int nitems, const ClusterInfo *__restrict__ cl,
+    const ClusterCtx *__restrict__ cx, const int *__restrict__ active,
+    Fields s, const double *__restrict__ orig, double *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
+  const int nwaves = gridDim.x * (kThreads / 64);
+  Soa soa{};
+  for (int k = 0; k < 8; ++k) soa.f[k] = s.f[k];
+  const NoParams np{};
+  for (int it = wave; it < nitems; it += nwaves) {
+    const Item w = items[it];
+    if (!active[w.c]) continue;
+    const ClusterInfo ci = cl[w.c];
+    const int tbase = cx[w.c].tbase;
+    int li = w.i0 + lane;
+    li = li < ci.n ? li : ci.n - 1;
+    AJ32::I q = AJ32::load_i(soa, ci.off + li, np);
+    AJ32::A64 acc;
+    AJ32::zero(acc);
+    for (int j0 = w.j0; j0 < w.j1; j0 += kTile) {
+      const int nk = min(kTile, w.j1 - j0);
+      AJ32::tile_begin(q, orig, tbase + j0 / kTile);
+      AJ32::A32 a;
+      AJ32::zero(a);
+      uint32_t near = 0u;
+      const AJ32::JRec *jr = rec + ci.off + j0;
+      // the record of j+1 is fetched while j is computed; whole tiles
+      // unrolled so that the fetch is in flight over a full pair
+      AJ32::JRec r = jr[0];
+      if (nk == kTile) {
+#pragma unroll 2
+        for (int k = 0; k < kTile; ++k) {
+          const AJ32::JRec rn = jr[min(k + 1, kTile - 1)];
+          near |= near_bit(AJ32::pair32(q, r, a)) << k;
+          r = rn;
+        }
+      } else {
+        for (int k = 0; k < nk; ++k) {
+          const AJ32::JRec rn = jr[min(k + 1, nk - 1)];
+          near |= near_bit(AJ32::pair32(q, r, a)) << k;
+          r = rn;
+        }
+      }
+      AJ32::fold(a, acc);
+      while (near) {
+        const int k = __builtin_ctz(near);
+        near &= near - 1u;
+        AJ32::near_pair(q, jr[k], acc, soa, soa, ci.off + j0 + k);
+      }
+    }
+    if (w.i0 + lane < ci.n) {
+      double v[6];
+      AJ32::values(acc, v);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        part[ci.base + ((long long)w.chunk * 6 + k) * ci.n + w.i0 + lane] =
+            v[k];
+    }
+  }
+}
+
 // Block step of each active cluster from min |tstep| over its particles:
 // get_base_tstep then get_min_block_tstep (integrator/__init__.py:48-78).
 // One block per cluster; the minimum is exact, so the reduction order does
@@ -389,6 +523,53 @@ __global__ void ens_init_flags(int ncl, const double *tc, double t_end,
   pairs[0] = 0.0;
 }
 
+// Spreads the low 21 bits of x to every third bit.
+inline uint64_t spread3(uint64_t x) {
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
+// perm[off .. off+n) = the cluster's input indices, sorted by the Morton
+// code of their positions (21 bits per axis over the cluster's bounding
+// box; ties and non-finite coordinates keep input order) when `morton`,
+// else in input order.  Nearby particles then share waves and j tiles, so
+// the FP32C near-field re-evaluations concentrate in a few tiles.
+void morton_order(const double *x, const double *y, const double *z, int off,
+                  int n, bool morton, int *perm) {
+  for (int k = 0; k < n; ++k) perm[off + k] = off + k;
+  if (!morton || n < 2) return;
+  const double *a[3] = {x, y, z};
+  double lo[3], sc[3];
+  for (int d = 0; d < 3; ++d) {
+    double mn = INFINITY, mx = -INFINITY;
+    for (int k = off; k < off + n; ++k)
+      if (std::isfinite(a[d][k])) {
+        mn = std::fmin(mn, a[d][k]);
+        mx = std::fmax(mx, a[d][k]);
+      }
+    lo[d] = mn;
+    sc[d] = (mx > mn) ? 2097151.0 / (mx - mn) : 0.0;
+  }
+  std::vector<uint64_t> code(n);
+  for (int k = 0; k < n; ++k) {
+    uint64_t c = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double v = a[d][off + k];
+      const double q = std::isfinite(v) ? (v - lo[d]) * sc[d] : 0.0;
+      const uint64_t b = q > 0.0 ? (uint64_t)std::fmin(q, 2097151.0) : 0;
+      c |= spread3(b) << d;
+    }
+    code[k] = c;
+  }
+  std::stable_sort(perm + off, perm + off + n,
+                   [&](int p, int q) { return code[p - off] < code[q - off]; });
+}
+
 inline int blocks_for(long n) {
   long g = (n + kThreads - 1) / kThreads;
   return (int)(g < 8192 ? (g > 0 ? g : 1) : 8192);
@@ -402,7 +583,12 @@ using namespace tupan;
 struct tupan_ensemble {
   int ncl = 0, ntot = 0;
   double eta = 0, t_end = 0;
-  std::vector<int> n, off;
+  std::vector<int> n, off, tbase;
+  // storage order: slot k of the device fields holds input particle perm[k]
+  // (each cluster's particles in Morton order of their positions at
+  // set_state, or in input order)
+  int order = TUPAN_ENS_ORDER_MORTON;
+  std::vector<int> perm;
   hipStream_t st = nullptr;
   // device
   double *fields[NFIELD] = {};
@@ -416,6 +602,13 @@ struct tupan_ensemble {
   double *tc = nullptr, *dt = nullptr, *pairs = nullptr;
   int *active = nullptr, *count = nullptr, *err = nullptr;
   long long *nsteps = nullptr;
+  // FP32C AccJerk (mode TUPAN_MODE_FP32C)
+  int mode = TUPAN_MODE_FP64;
+  int ntiles = 0;
+  AccJerkF32c::JRec *rec32 = nullptr;
+  double *orig = nullptr;
+  ClusterCtx *cctx = nullptr;
+  int *tcid = nullptr;
   // launch shape (autotuner)
   int ipl = 1, jchunk = 512, waves_per_simd = 8, blocks = 0;
   bool built = false;
@@ -446,7 +639,8 @@ void ens_free_plan(tupan_ensemble *e) {
 // Work items and partial-sum layout for the current launch shape.
 int ens_plan(tupan_ensemble *e) {
   ens_free_plan(e);
-  const int P = e->ipl;
+  // the FP32C pair kernel takes one i-particle per lane
+  const int P = e->mode == TUPAN_MODE_FP32C ? 1 : e->ipl;
   std::vector<ClusterInfo> cl(e->ncl);
   std::vector<Item> items;
   long long base = 0;
@@ -494,7 +688,8 @@ int ens_plan(tupan_ensemble *e) {
 
 template <class Pol>
 void launch_pairs(tupan_ensemble *e, typename Pol::Params prm) {
-  if (e->ipl == 2)
+  // the FP32C plan holds items of 64 i's (one per lane)
+  if (e->ipl == 2 && e->mode != TUPAN_MODE_FP32C)
     hipLaunchKernelGGL((ens_pairs<Pol, 2>), dim3(e->blocks), dim3(kThreads),
                        0, e->st, e->rec, e->items, e->nitems, e->cl,
                        e->active, e->part, prm);
@@ -522,11 +717,24 @@ void ens_enqueue_step(tupan_ensemble *e) {
   hipLaunchKernelGGL(ens_choose_dt, dim3(e->ncl), dim3(kThreads), 0, e->st,
                      e->cl, e->fields[FTS], e->tc, e->active, e->dt, e->eta,
                      e->t_end, e->err);
+  const bool f32c = e->mode == TUPAN_MODE_FP32C;
   for (int pass = 0; pass < 3; ++pass) {
-    if (pass > 0)
-      hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st,
-                         e->ntot, f, e->cid, e->active, e->rec);
-    launch_pairs<AJ>(e, ap);
+    if (f32c) {
+      hipLaunchKernelGGL(ens_origins, dim3(blocks_for(e->ntiles)),
+                         dim3(kThreads), 0, e->st, e->ntiles, e->tcid, e->cl,
+                         e->cctx, e->active, f, e->orig);
+      hipLaunchKernelGGL(ens_pack32, dim3(gb), dim3(kThreads), 0, e->st,
+                         e->ntot, f, e->cid, e->cl, e->cctx, e->active,
+                         e->orig, e->rec32);
+      hipLaunchKernelGGL(ens_pairs32, dim3(e->blocks), dim3(kThreads), 0,
+                         e->st, e->rec32, e->items, e->nitems, e->cl,
+                         e->cctx, e->active, f, e->orig, e->part);
+    } else {
+      if (pass > 0)
+        hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st,
+                           e->ntot, f, e->cid, e->active, e->rec);
+      launch_pairs<AJ>(e, ap);
+    }
     hipLaunchKernelGGL((ens_finish<AJ>), dim3(gb), dim3(kThreads), 0, e->st,
                        e->ntot, e->cid, e->cl, e->active, e->part, aj, ap);
     if (pass == 0)
@@ -593,6 +801,14 @@ int tupan_ensemble_create(tupan_ensemble **out, int ncl, const int *sizes,
   std::vector<int> cid(e->ntot);
   for (int c = 0; c < ncl; ++c)
     std::fill(cid.begin() + e->off[c], cid.begin() + e->off[c] + e->n[c], c);
+  // tiles numbered within each cluster, clusters back to back
+  std::vector<int> tcid;
+  e->tbase.resize(ncl);
+  for (int c = 0; c < ncl; ++c) {
+    e->tbase[c] = (int)tcid.size();
+    tcid.insert(tcid.end(), (e->n[c] + kTile - 1) / kTile, c);
+  }
+  e->ntiles = (int)tcid.size();
   const size_t nt = (size_t)e->ntot;
   if ((er = hipMalloc(&e->slab, sizeof(double) * nt * NFIELD)) !=
           hipSuccess ||
@@ -604,7 +820,13 @@ int tupan_ensemble_create(tupan_ensemble **out, int ncl, const int *sizes,
       (er = hipMalloc(&e->nsteps, sizeof(long long) * ncl)) != hipSuccess ||
       (er = hipMalloc(&e->pairs, sizeof(double))) != hipSuccess ||
       (er = hipMalloc(&e->count, sizeof(int))) != hipSuccess ||
-      (er = hipMalloc(&e->err, sizeof(int))) != hipSuccess) {
+      (er = hipMalloc(&e->err, sizeof(int))) != hipSuccess ||
+      (er = hipMalloc(&e->rec32, sizeof(AccJerkF32c::JRec) * nt)) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->orig, sizeof(double) * 4 * e->ntiles)) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->cctx, sizeof(ClusterCtx) * ncl)) != hipSuccess ||
+      (er = hipMalloc(&e->tcid, sizeof(int) * e->ntiles)) != hipSuccess) {
     int rc = hip_fail(er, "ensemble allocation");
     tupan_ensemble_destroy(e);
     return rc;
@@ -616,6 +838,9 @@ int tupan_ensemble_create(tupan_ensemble **out, int ncl, const int *sizes,
   (void)hipMemsetAsync(e->nsteps, 0, sizeof(long long) * ncl, e->st);
   er = hipMemcpyAsync(e->cid, cid.data(), sizeof(int) * nt,
                       hipMemcpyHostToDevice, e->st);
+  if (er == hipSuccess)
+    er = hipMemcpyAsync(e->tcid, tcid.data(), sizeof(int) * e->ntiles,
+                        hipMemcpyHostToDevice, e->st);
   if (er == hipSuccess) er = hipStreamSynchronize(e->st);
   if (er != hipSuccess) {
     int rc = hip_fail(er, "ensemble init");
@@ -630,8 +855,10 @@ int tupan_ensemble_destroy(tupan_ensemble *e) {
   if (!e) return TUPAN_OK;
   if (e->st) (void)hipStreamSynchronize(e->st);
   ens_free_plan(e);
-  void *ptrs[] = {e->slab, e->rec, e->cid,   e->tc,  e->dt,
-                  e->active, e->nsteps, e->pairs, e->count, e->err};
+  void *ptrs[] = {e->slab,   e->rec,    e->cid,   e->tc,
+                  e->dt,     e->active, e->nsteps, e->pairs,
+                  e->count,  e->err,    e->rec32, e->orig,
+                  e->cctx,   e->tcid};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (e->st) (void)hipStreamDestroy(e->st);
@@ -646,6 +873,9 @@ int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
       waves_per_simd > 32)
     return set_error(TUPAN_ERR_ARG, "ensemble launch: ipl 1|2, jchunk >= 16,"
                                     " 1 <= waves_per_simd <= 32");
+  if (e->mode == TUPAN_MODE_FP32C && jchunk % kTile != 0)
+    return set_error(TUPAN_ERR_ARG, "FP32C needs a j chunk that is a "
+                                    "multiple of %d", kTile);
   (void)hipStreamSynchronize(e->st);
   e->ipl = ipl;
   e->jchunk = jchunk;
@@ -654,17 +884,77 @@ int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
   return TUPAN_OK;
 }
 
+int tupan_ensemble_set_mode(tupan_ensemble *e, int mode) {
+  if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
+  if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
+    return set_error(TUPAN_ERR_ARG, "ensemble mode: FP64 or FP32C");
+  if (mode == TUPAN_MODE_FP32C && e->jchunk % kTile != 0)
+    return set_error(TUPAN_ERR_ARG, "FP32C needs a j chunk that is a "
+                                    "multiple of %d", kTile);
+  (void)hipStreamSynchronize(e->st);
+  e->mode = mode;
+  ens_free_plan(e);
+  return TUPAN_OK;
+}
+
 // fields: m, rx, ry, rz, e2, vx, vy, vz (host, ntot each); t0: per-cluster
 // start times (may be NULL = 0).  Resets the step counters.
+int tupan_ensemble_set_order(tupan_ensemble *e, int order) {
+  if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
+  if (order != TUPAN_ENS_ORDER_INPUT && order != TUPAN_ENS_ORDER_MORTON)
+    return set_error(TUPAN_ERR_ARG, "ensemble order: INPUT or MORTON");
+  e->order = order;
+  return TUPAN_OK;
+}
+
 int tupan_ensemble_set_state(tupan_ensemble *e, const double *const *fields,
                              const double *t0) {
   if (!e || !fields) return set_error(TUPAN_ERR_ARG, "NULL argument");
   const size_t nt = (size_t)e->ntot;
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < 8; ++k)
     if (!fields[k]) return set_error(TUPAN_ERR_ARG, "NULL field %d", k);
-    hipError_t er = hipMemcpyAsync(e->fields[k], fields[k],
-                                   sizeof(double) * nt,
+  e->perm.resize(nt);
+  for (int c = 0; c < e->ncl; ++c)
+    morton_order(fields[FX], fields[FY], fields[FZ], e->off[c], e->n[c],
+                 e->order == TUPAN_ENS_ORDER_MORTON, e->perm.data());
+  {
+    std::vector<double> stage(8 * nt);
+    for (int k = 0; k < 8; ++k)
+      for (size_t i = 0; i < nt; ++i)
+        stage[k * nt + i] = fields[k][e->perm[i]];
+    for (int k = 0; k < 8; ++k) {
+      hipError_t er = hipMemcpyAsync(e->fields[k], stage.data() + k * nt,
+                                     sizeof(double) * nt,
+                                     hipMemcpyHostToDevice, e->st);
+      if (er != hipSuccess) return hip_fail(er, "ensemble upload");
+    }
+    // stage must outlive the copies
+    hipError_t er = hipStreamSynchronize(e->st);
+    if (er != hipSuccess) return hip_fail(er, "ensemble upload");
+  }
+  // FP32C near-field context of each cluster (the single-system rule,
+  // ctx_final_kernel: factor = c^2 n^(1/3) / sum m, largest e2)
+  std::vector<ClusterCtx> cx(e->ncl);
+  const double cnear = near_field_factor();
+  for (int c = 0; c < e->ncl; ++c) {
+    double msum = 0.0, e2max = 0.0;
+    for (int i = e->off[c]; i < e->off[c] + e->n[c]; ++i) {
+      msum += fields[FM][i];
+      e2max = std::fmax(e2max, fields[FE2][i]);
+    }
+    const double f = cnear * cnear * std::cbrt((double)e->n[c]) / msum;
+    cx[c].factor = (cnear > 0.0 && msum > 0.0 && f < 1e300) ? f : 0.0;
+    cx[c].e2max = e2max;
+    cx[c].tbase = e->tbase[c];
+    cx[c].pad = 0;
+  }
+  {
+    hipError_t er = hipMemcpyAsync(e->cctx, cx.data(),
+                                   sizeof(ClusterCtx) * e->ncl,
                                    hipMemcpyHostToDevice, e->st);
+    if (er != hipSuccess) return hip_fail(er, "ensemble upload");
+    // cx must outlive the copy
+    er = hipStreamSynchronize(e->st);
     if (er != hipSuccess) return hip_fail(er, "ensemble upload");
   }
   std::vector<double> t(e->ncl, 0.0);
@@ -686,11 +976,19 @@ int tupan_ensemble_get_state(tupan_ensemble *e, double *const *out,
                              double *tc, long long *nsteps) {
   if (!e) return set_error(TUPAN_ERR_ARG, "NULL ensemble");
   const size_t nt = (size_t)e->ntot;
+  if (out && e->perm.size() != nt)
+    return set_error(TUPAN_ERR_ARG, "ensemble: no state set");
   hipError_t er = hipSuccess;
+  std::vector<double> stage(out ? nt : 0);
   for (int k = 0; out && k < 6 && er == hipSuccess; ++k)
-    if (out[k])
-      er = hipMemcpyAsync(out[k], e->fields[k < 3 ? FX + k : FVX + (k - 3)],
+    if (out[k]) {
+      er = hipMemcpyAsync(stage.data(),
+                          e->fields[k < 3 ? FX + k : FVX + (k - 3)],
                           sizeof(double) * nt, hipMemcpyDeviceToHost, e->st);
+      if (er == hipSuccess) er = hipStreamSynchronize(e->st);
+      if (er == hipSuccess)
+        for (size_t i = 0; i < nt; ++i) out[k][e->perm[i]] = stage[i];
+    }
   if (er == hipSuccess && tc)
     er = hipMemcpyAsync(tc, e->tc, sizeof(double) * e->ncl,
                         hipMemcpyDeviceToHost, e->st);

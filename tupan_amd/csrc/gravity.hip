@@ -24,6 +24,7 @@
 //               tupan/lib/src/tstep_kernel.c:46-47)
 // with the softening of tupan/lib/src/smoothing.h:10-184: s^2 = r^2 + e2_i +
 // e2_j, and a pair contributes nothing unless the *unsoftened* r^2 > 0.
+#include "accjerk.hpp"
 #include "f32c.hpp"
 #include "launch.hpp"
 
@@ -154,106 +155,6 @@ struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
     a.v[0] -= k * rx;
     a.v[1] -= k * ry;
     a.v[2] -= k * rz;
-  }
-};
-
-// ============================================================ AccJerk
-struct AccJerkF64 : SumOut<6> {
-  static constexpr bool kTwoLevel = false;
-  static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
-  using Params = NoParams;
-  struct JRec { double m, x, y, z, e2, vx, vy, vz; };
-  struct I { double x, y, z, e2, vx, vy, vz; };
-  using A64 = F64Acc<6>;
-  __device__ static JRec pack(const Soa &s, int j, double) {
-    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
-            s.f[E2][j], s.f[VX][j], s.f[VY][j], s.f[VZ][j]};
-  }
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
-    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i],
-            s.f[VX][i], s.f[VY][i], s.f[VZ][i]};
-  }
-  __device__ static void zero(A64 &a) { zero_acc(a); }
-  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
-  __device__ static void pair(const I &q, const JRec &r, A64 &a,
-                              const Params &) {
-    double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
-    double vx = q.vx - r.vx, vy = q.vy - r.vy, vz = q.vz - r.vz;
-    double r2 = rx * rx + ry * ry + rz * rz;
-    double rv = rx * vx + ry * vy + rz * vz;
-    double y = rsq64(r2 + (q.e2 + r.e2));
-    y = r2 > 0.0 ? y : 0.0;
-    double y2 = y * y;
-    double k = r.m * y * y2;
-    double alpha = 3.0 * rv * y2;
-    vx -= alpha * rx;
-    vy -= alpha * ry;
-    vz -= alpha * rz;
-    a.v[0] -= k * rx;
-    a.v[1] -= k * ry;
-    a.v[2] -= k * rz;
-    a.v[3] -= k * vx;
-    a.v[4] -= k * vy;
-    a.v[5] -= k * vz;
-  }
-};
-
-struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
-  static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
-  // Velocity differences from FP32 offsets to the j tile's FP64 velocity
-  // origin: the j offset is rounded once in pack, the i offset once per
-  // tile (tile_begin).  Rounding v_i itself to FP32 would add the same error
-  // to every pair of particle i, and that coherent term
-  // (dv_i * sum_j m_j/s^3) would not average out in the jerk; per tile it
-  // is shared by 32 pairs only.
-  static constexpr int kVelOrigins = 1;
-  struct JRec { double x, y, z; float dvx, dvy, dvz, m, e2, near2; };
-  struct I {
-    double x, y, z, vx, vy, vz;
-    float e2, dvx, dvy, dvz;
-    int i;
-  };
-  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
-    const double *o = c.origins + 4 * (j / kTile);
-    return {s.f[RX][j],
-            s.f[RY][j],
-            s.f[RZ][j],
-            dsub32(s.f[VX][j], o[0]),
-            dsub32(s.f[VY][j], o[1]),
-            dsub32(s.f[VZ][j], o[2]),
-            (float)s.f[M][j],
-            (float)s.f[E2][j],
-            near_threshold(s.f[M][j], s.f[E2][j], c)};
-  }
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
-    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i], s.f[VY][i],
-            s.f[VZ][i], (float)s.f[E2][i], 0.0f, 0.0f, 0.0f, i};
-  }
-  // the i-side velocity offsets to tile t's origin
-  __device__ static void tile_begin(I &q, const double *origins, int t) {
-    const double *o = origins + 4 * t;
-    q.dvx = dsub32(q.vx, o[0]);
-    q.dvy = dsub32(q.vy, o[1]);
-    q.dvz = dsub32(q.vz, o[2]);
-  }
-  __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
-                                float rz, float s2, bool far, A32 &a) {
-    float vx = q.dvx - r.dvx, vy = q.dvy - r.dvy, vz = q.dvz - r.dvz;
-    float rv = rx * vx + ry * vy + rz * vz;
-    float y = rsq32(s2);
-    y = far ? y : 0.0f;
-    float y2 = y * y;
-    float k = r.m * y * y2;
-    float alpha = 3.0f * rv * y2;
-    vx -= alpha * rx;
-    vy -= alpha * ry;
-    vz -= alpha * rz;
-    a.v[0] -= k * rx;
-    a.v[1] -= k * ry;
-    a.v[2] -= k * rz;
-    a.v[3] -= k * vx;
-    a.v[4] -= k * vy;
-    a.v[5] -= k * vz;
   }
 };
 

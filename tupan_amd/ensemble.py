@@ -39,7 +39,9 @@ class Ensemble(object):
     """``sizes``: particles per cluster (clusters are concatenated in this
     order in every field array)."""
 
-    def __init__(self, sizes, eta, t_end):
+    ORDERS = {"input": 0, "morton": 1}
+
+    def __init__(self, sizes, eta, t_end, mode="fp64", order="morton"):
         self.sizes = np.asarray(sizes, dtype=np.int32)
         if self.sizes.ndim != 1 or len(self.sizes) == 0:
             raise ValueError("sizes must be a non-empty 1-D list")
@@ -53,6 +55,30 @@ class Ensemble(object):
         _lib.check(_lib.lib().tupan_ensemble_create(
             ctypes.byref(h), len(self.sizes), arr, self.eta, self.t_end))
         self._h = h
+        self.mode = "fp64"
+        if mode != "fp64":
+            self.set_mode(mode)
+        self.set_order(order)
+
+    def set_order(self, order):
+        """Device storage order of each cluster's particles from the next
+        set_state: 'morton' (neighbours share waves and j tiles, which
+        the FP32C near field needs) or 'input'.  Results come back in
+        input order either way."""
+        if order not in self.ORDERS:
+            raise ValueError("ensemble order: 'morton' or 'input'")
+        _lib.check(_lib.lib().tupan_ensemble_set_order(self._h,
+                                                       self.ORDERS[order]))
+        self.order = order
+
+    def set_mode(self, mode):
+        """'fp64' or 'fp32c' (AccJerk sums in FP32C arithmetic; one
+        i-particle per lane)."""
+        if mode not in ("fp64", "fp32c"):
+            raise ValueError("ensemble mode: 'fp64' or 'fp32c'")
+        _lib.check(_lib.lib().tupan_ensemble_set_mode(self._h,
+                                                      _lib.MODES[mode]))
+        self.mode = mode
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -135,7 +161,8 @@ class Ensemble(object):
         import torch
         name = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "?"
         hist = np.bincount(np.minimum(np.log2(self.sizes).astype(int), 30))
-        key = "%s|%d|%s" % (name, len(self.sizes), ",".join(map(str, hist)))
+        key = "%s|%s|%d|%s" % (name, self.mode, len(self.sizes),
+                               ",".join(map(str, hist)))
         return hashlib.sha1(key.encode()).hexdigest()[:16], key
 
     def autotune(self, fields, trial_steps=8, candidates=None, use_cache=True):
@@ -156,7 +183,10 @@ class Ensemble(object):
             self.set_state(fields)
             return best, cache[sig].get("rates", {})
         rates = {}
-        for cand in candidates or CANDIDATES:
+        cands = candidates or CANDIDATES
+        if self.mode == "fp32c":   # one i-particle per lane
+            cands = sorted({(1, jc, w) for _, jc, w in cands})
+        for cand in cands:
             self.set_launch(*cand)
             self.set_state(fields)
             self.step(1)            # plan, capture, warm up
