@@ -89,6 +89,66 @@ struct VelOrigins<Pol, decltype((void)Pol::kVelOrigins)> {
   static constexpr int value = Pol::kVelOrigins;
 };
 
+// FP32C calls with at most this many j's (and kSmallCtxI i's) build their
+// context, tile origins and records in one single-block launch
+// (small_pack_kernel) instead of four: at these sizes each launch costs more
+// than its work.
+constexpr int kSmallCtxJ = 4096;
+constexpr int kSmallCtxI = 65536;
+
+// ctx_partial_kernel + ctx_final_kernel + tile_origin_kernel + pack_kernel
+// in one block of 256 threads: sum of the j masses and largest i-side e2
+// (tree-reduced in LDS, a fixed order), the near-field factor, the per-tile
+// origins (written to ctx + 2 for the pair kernel and kept in LDS for the
+// packing), then the records.  nj <= kSmallCtxJ.
+template <class Pol>
+__global__ __launch_bounds__(256) void small_pack_kernel(
+    Soa is, int ni, Soa js, int nj, double c, typename Pol::JRec *jr,
+    double *ctx) {
+  __shared__ double ps[256], px[256];
+  __shared__ double org[4 * (kSmallCtxJ / kTile)];
+  const int t = threadIdx.x;
+  double s = 0.0, x = 0.0;
+  for (int j = t; j < nj; j += 256) s += js.f[M][j];
+  for (int i = t; i < ni; i += 256) x = fmax(x, is.f[E2][i]);
+  ps[t] = s;
+  px[t] = x;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      ps[t] += ps[t + w];
+      px[t] = fmax(px[t], px[t + w]);
+    }
+    __syncthreads();
+  }
+  const double msum = ps[0], e2max = px[0];
+  const double f = c * c * cbrt((double)nj) / msum;
+  const double factor = (c > 0.0 && msum > 0.0 && f < 1e300) ? f : 0.0;
+  if (VelOrigins<Pol>::value) {
+    const int ntiles = (nj + kTile - 1) / kTile;
+    for (int tt = t; tt < ntiles; tt += 256) {
+      const int j0 = tt * kTile, je = min(nj, j0 + kTile);
+      for (int k = 0; k < 3; ++k) {
+        const double *v = js.f[VX + k];
+        double lo = v[j0], hi = lo;
+        for (int j = j0 + 1; j < je; ++j) {
+          lo = fmin(lo, v[j]);
+          hi = fmax(hi, v[j]);
+        }
+        org[4 * tt + k] = 0.5 * (lo + hi);
+        ctx[2 + 4 * tt + k] = 0.5 * (lo + hi);
+      }
+    }
+  }
+  if (t == 0) {
+    ctx[0] = factor;
+    ctx[1] = e2max;
+  }
+  __syncthreads();
+  for (int j = t; j < nj; j += 256)
+    jr[j] = Pol::pack(js, j, PackCtx{factor, e2max, org});
+}
+
 template <class Pol>
 int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
                size_t nj, const double *const *jp, double *const *op,
@@ -159,7 +219,13 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
       return rc;
     }
   }
-  if (n_j > 0) {
+  const bool small = Pol::kTwoLevel && n_j <= kSmallCtxJ &&
+                     n_i <= kSmallCtxI;
+  if (n_j > 0 && small) {
+    if constexpr (Pol::kTwoLevel)
+      hipLaunchKernelGGL((small_pack_kernel<Pol>), dim3(1), dim3(256), 0, st,
+                         is, n_i, js, n_j, near_field_factor(), jr, ctx);
+  } else if (n_j > 0) {
     if (Pol::kTwoLevel) {
       double *scratch = ctx + 2 + 4 * ntiles;
       hipLaunchKernelGGL(ctx_partial_kernel, dim3(kCtxBlocks), dim3(256), 0,
