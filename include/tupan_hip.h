@@ -289,12 +289,13 @@ int tupan_phi_adjoint_dev(int mode, size_t ni, const double *const *ip,
  * hermite.py:340-410 with the block steps of tupan/integrator/
  * __init__.py:48-78) on its own clock, and the ensemble batches the pair
  * sums of all clusters into one persistent kernel per phase.  One ensemble
- * step (Tstep, block-step choice, AccJerk + predict, 2 x (AccJerk +
- * correct), clock) is captured once into a HIP graph and replayed.
- * FP64 arithmetic by default; tupan_ensemble_set_mode(TUPAN_MODE_FP32C)
- * evaluates the AccJerk sums in the FP32C arithmetic of the single-system
- * kernels (near-field factor and velocity tile origins per cluster), the
- * Tstep sums and the updates stay FP64.  Device memory is allocated at
+ * step (Tstep and AccJerk at the start state in one pair pass, block-step
+ * choice, predict, 2 x (AccJerk + correct), clock) is captured once into a
+ * HIP graph and replayed.  FP64 arithmetic by default;
+ * tupan_ensemble_set_mode(TUPAN_MODE_FP32C) evaluates the two corrector
+ * AccJerk sums in the FP32C arithmetic of the single-system kernels
+ * (near-field factor and velocity tile origins per cluster); the start
+ * pass (Tstep with AccJerk) and the updates stay FP64.  Device memory is allocated at
  * create and at the first step after set_launch or set_mode; steps
  * allocate nothing. */
 typedef struct tupan_ensemble tupan_ensemble;
@@ -306,8 +307,8 @@ int tupan_ensemble_destroy(tupan_ensemble *e);
  * item, resident waves per SIMD of the persistent kernels. */
 int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
                               int waves_per_simd);
-/* mode = TUPAN_MODE_FP64 (default) or TUPAN_MODE_FP32C (AccJerk only; one
- * i-particle per lane, jchunk a multiple of 32). */
+/* mode = TUPAN_MODE_FP64 (default) or TUPAN_MODE_FP32C (the corrector
+ * AccJerk passes; one i-particle per lane there, jchunk a multiple of 32). */
 int tupan_ensemble_set_mode(tupan_ensemble *e, int mode);
 /* Storage order of each cluster's particles on the device, applied at the
  * next set_state (get_state returns input order either way):

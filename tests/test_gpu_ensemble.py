@@ -9,7 +9,7 @@ component's scale (the pair sums are split differently, so only rounding
 differs).  The captured-graph replay and the launched-kernel path give
 identical bits, and the autotuner's launch shapes agree to 1e-11.
 
-FP32C mode (AccJerk sums in FP32C arithmetic): every cluster takes the
+FP32C mode (corrector AccJerk sums in FP32C arithmetic): every cluster takes the
 same steps as in FP64 and ends within 1e-7 of each component's scale; the
 graph replay is bitwise the launched kernels; j-chunk lengths change only
 the FP64 grouping of the tile sums (1e-11); bad modes and j chunks that are

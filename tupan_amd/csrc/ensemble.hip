@@ -6,9 +6,9 @@
 // tupan/integrator/hermite.py:340-410 with the block steps of
 // tupan/integrator/__init__.py:48-78) on its own clock t_c:
 //
-//   tstep  <- Tstep(c, c, eta)                      (FP64 pair sums)
+//   tstep, a0, j0 <- Tstep(c, c, eta), AccJerk(c, c)   (one pair pass)
 //   tau    <- block step from min |tstep| and min(t_end - t_c, eta)
-//   snapshot, a0, j0 <- AccJerk(c, c); predict
+//   snapshot; predict
 //   twice: a1, j1 <- AccJerk(c, c); correct
 //   t_c += tau
 //
@@ -23,7 +23,9 @@
 // resident waves, so that 64-particle and 4096-particle clusters share the
 // chip evenly.  Each item writes FP64 partial sums; a finishing kernel adds a
 // cluster's chunks in chunk order (bit-reproducible).  The pair arithmetic
-// is the single-evaluation FP64 policies' (gravity.hip).
+// is the single-evaluation FP64 policies' (gravity.hip); the Tstep and
+// AccJerk sums at the start state share one pass (TSAJ), and in FP32C mode
+// the two corrector AccJerk passes use the FP32C policy (accjerk.hpp).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -99,49 +101,78 @@ struct AJ {
   }
 };
 
-struct TS {
-  static constexpr int kOut = 2;
-  struct A { double sum, mx; };
+// Tstep and AccJerk of one pair together (the two evaluations at the start
+// of a step share the state): the Tstep terms first (gravity.hip's TstepF64,
+// tstep_kernel_common.h:7-56), then AJ's, each with the same operations as
+// its own kernel; the shared 1/sqrt(r^2 + e2) is the same value in both.
+// Outputs: tstep, tstepij, ax..jz.
+struct TSAJ {
+  static constexpr int kOut = 8;
+  struct A { double v[8]; };
   struct Params { double eta; };
-  __device__ static void zero(A &a) { a.sum = 0.0; a.mx = 0.0; }
+  __device__ static void zero(A &a) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.v[k] = 0.0;
+  }
   __device__ static void pair(const AccJerkF64Rec &q, const AccJerkF64Rec &r,
                               A &a, const Params &prm) {
     double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
     double vx = q.vx - r.vx, vy = q.vy - r.vy, vz = q.vz - r.vz;
-    double m = q.m + r.m;
     double r2 = rx * rx + ry * ry + rz * rz;
     double rv = rx * vx + ry * vy + rz * vz;
-    double v2 = vx * vx + vy * vy + vz * vz;
     const bool real_pair = r2 > 0.0;
-    double inv_r1 = rsq64(r2 + (q.e2 + r.e2));
-    inv_r1 = real_pair ? inv_r1 : 0.0;
-    double inv_r2 = inv_r1 * inv_r1;
-    double phi = m * inv_r1;
-    double w2 = inv_r2 * (v2 + 2.0 * phi);
-    double gam = inv_r2 * (w2 + 2.0 * inv_r2 * phi);
-    gam *= prm.eta * rsq64(w2);
-    w2 -= gam * rv;
-    w2 = real_pair ? w2 : 0.0;
-    a.sum += w2;
-    a.mx = fmax(w2, a.mx);
+    double y = rsq64(r2 + (q.e2 + r.e2));
+    y = real_pair ? y : 0.0;
+    double y2 = y * y;
+    // Tstep
+    {
+      double v2 = vx * vx + vy * vy + vz * vz;
+      double phi = (q.m + r.m) * y;
+      double w2 = y2 * (v2 + 2.0 * phi);
+      double gam = y2 * (w2 + 2.0 * y2 * phi);
+      gam *= prm.eta * rsq64(w2);
+      w2 -= gam * rv;
+      w2 = real_pair ? w2 : 0.0;
+      a.v[0] += w2;
+      a.v[1] = fmax(w2, a.v[1]);
+    }
+    // AccJerk (AJ::pair)
+    double k = r.m * y * y2;
+    double alpha = 3.0 * rv * y2;
+    vx -= alpha * rx;
+    vy -= alpha * ry;
+    vz -= alpha * rz;
+    a.v[2] -= k * rx;
+    a.v[3] -= k * ry;
+    a.v[4] -= k * rz;
+    a.v[5] -= k * vx;
+    a.v[6] -= k * vy;
+    a.v[7] -= k * vz;
   }
   __device__ static void values(const A &a, double *v) {
-    v[0] = a.sum;
-    v[1] = a.mx;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = a.v[k];
   }
   __device__ static void combine(double *v, const double *w) {
     v[0] += w[0];
     v[1] = fmax(w[1], v[1]);
+#pragma unroll
+    for (int k = 2; k < 8; ++k) v[k] += w[k];
   }
   __device__ static void finish(const double *v, double *const *out, int i,
                                 const Params &prm) {
     out[0][i] = prm.eta / sqrt(1.0 + v[0]);
     out[1][i] = prm.eta / sqrt(1.0 + v[1]);
+#pragma unroll
+    for (int k = 2; k < 8; ++k) out[k][i] = v[k];
   }
 };
 
+// Largest number of outputs of a pair policy (partials per chunk).
+constexpr int kMaxOut = 8;
+
 struct Outs {
-  double *f[6];
+  double *f[kMaxOut];
 };
 
 // Persistent pair kernel.  Wave w handles items w, w + W, w + 2W, ... of
@@ -648,7 +679,7 @@ int ens_plan(tupan_ensemble *e) {
     const int nc = e->n[c];
     const int nchunk = std::max(1, (nc + e->jchunk - 1) / e->jchunk);
     cl[c] = {e->off[c], nc, nchunk, base};
-    base += (long long)nchunk * 6 * nc;
+    base += (long long)nchunk * kMaxOut * nc;
     for (int i0 = 0; i0 < nc; i0 += 64 * P)
       for (int s = 0; s < nchunk; ++s) {
         const int j0 = s * e->jchunk;
@@ -704,21 +735,26 @@ void ens_enqueue_step(tupan_ensemble *e) {
   Fields f;
   for (int k = 0; k < NFIELD; ++k) f.f[k] = e->fields[k];
   const int gb = blocks_for(e->ntot);
-  Outs ts{{e->fields[FTS], e->fields[FTSIJ]}};
+  Outs tsaj{{e->fields[FTS], e->fields[FTSIJ], e->fields[FAX],
+             e->fields[FAY], e->fields[FAZ], e->fields[FJX], e->fields[FJY],
+             e->fields[FJZ]}};
   Outs aj{{e->fields[FAX], e->fields[FAY], e->fields[FAZ], e->fields[FJX],
            e->fields[FJY], e->fields[FJZ]}};
-  TS::Params tp{e->eta};
+  TSAJ::Params tp{e->eta};
   AJ::Params ap{};
+  // Tstep and the derivatives at the step's start state: one pass
   hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st, e->ntot,
                      f, e->cid, e->active, e->rec);
-  launch_pairs<TS>(e, tp);
-  hipLaunchKernelGGL((ens_finish<TS>), dim3(gb), dim3(kThreads), 0, e->st,
-                     e->ntot, e->cid, e->cl, e->active, e->part, ts, tp);
+  launch_pairs<TSAJ>(e, tp);
+  hipLaunchKernelGGL((ens_finish<TSAJ>), dim3(gb), dim3(kThreads), 0, e->st,
+                     e->ntot, e->cid, e->cl, e->active, e->part, tsaj, tp);
   hipLaunchKernelGGL(ens_choose_dt, dim3(e->ncl), dim3(kThreads), 0, e->st,
                      e->cl, e->fields[FTS], e->tc, e->active, e->dt, e->eta,
                      e->t_end, e->err);
+  hipLaunchKernelGGL(ens_predict, dim3(gb), dim3(kThreads), 0, e->st,
+                     e->ntot, f, e->cid, e->active, e->dt);
   const bool f32c = e->mode == TUPAN_MODE_FP32C;
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = 1; pass < 3; ++pass) {
     if (f32c) {
       hipLaunchKernelGGL(ens_origins, dim3(blocks_for(e->ntiles)),
                          dim3(kThreads), 0, e->st, e->ntiles, e->tcid, e->cl,
@@ -730,19 +766,14 @@ void ens_enqueue_step(tupan_ensemble *e) {
                          e->st, e->rec32, e->items, e->nitems, e->cl,
                          e->cctx, e->active, f, e->orig, e->part);
     } else {
-      if (pass > 0)
-        hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st,
-                           e->ntot, f, e->cid, e->active, e->rec);
+      hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st,
+                         e->ntot, f, e->cid, e->active, e->rec);
       launch_pairs<AJ>(e, ap);
     }
     hipLaunchKernelGGL((ens_finish<AJ>), dim3(gb), dim3(kThreads), 0, e->st,
                        e->ntot, e->cid, e->cl, e->active, e->part, aj, ap);
-    if (pass == 0)
-      hipLaunchKernelGGL(ens_predict, dim3(gb), dim3(kThreads), 0, e->st,
-                         e->ntot, f, e->cid, e->active, e->dt);
-    else
-      hipLaunchKernelGGL(ens_correct, dim3(gb), dim3(kThreads), 0, e->st,
-                         e->ntot, f, e->cid, e->active, e->dt);
+    hipLaunchKernelGGL(ens_correct, dim3(gb), dim3(kThreads), 0, e->st,
+                       e->ntot, f, e->cid, e->active, e->dt);
   }
   hipLaunchKernelGGL(ens_clock, dim3(1), dim3(kThreads), 0, e->st, e->ncl,
                      e->cl, e->tc, e->dt, e->active, e->nsteps, e->pairs,
