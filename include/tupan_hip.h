@@ -295,9 +295,9 @@ int tupan_phi_adjoint_dev(int mode, size_t ni, const double *const *ip,
  * tupan_ensemble_set_mode(TUPAN_MODE_FP32C) evaluates the two corrector
  * AccJerk sums in the FP32C arithmetic of the single-system kernels
  * (near-field factor and velocity tile origins per cluster); the start
- * pass (Tstep with AccJerk) and the updates stay FP64.  Device memory is allocated at
- * create and at the first step after set_launch or set_mode; steps
- * allocate nothing. */
+ * pass (Tstep with AccJerk) and the updates stay FP64.  Device memory is
+ * allocated at create and at the first step after set_launch or set_mode;
+ * steps allocate nothing. */
 typedef struct tupan_ensemble tupan_ensemble;
 /* sizes[c] = particles of cluster c; clusters are stored back to back. */
 int tupan_ensemble_create(tupan_ensemble **out, int nclusters,
