@@ -282,6 +282,53 @@ def test_launch_shapes_agree_and_runs_are_bitwise_reproducible():
         _lib.set_launch("acc_jerk", "fp32c", 0, 0)
 
 
+_CACHE = {}
+
+
+@pytest.mark.parametrize("ni,nj", [(300, 4096), (300, 4097), (65537, 200)])
+def test_fp32c_context_paths(ni, nj):
+    """FP32C builds its near-field context, tile origins and records in one
+    block for nj <= 4096 and ni <= 65536, else in four launches: both sides
+    of each bound stay within the mode's error of FP64, per particle (Phi
+    and acc 1e-6; jerk RMS 3e-7, worst 3e-5; snap RMS 1e-6, worst 1e-4;
+    crackle RMS 1e-5, worst 1e-3), for every FP32C kernel."""
+    if "ctx" not in _CACHE:
+        ps = _plummer(65537, 8, 0.002)
+        d = {f: getattr(ps, f) for f in spec.F8}
+        aj = kernels.compute("acc_jerk", d, d, mode="fp64")
+        _CACHE["ctx"] = d, dict(zip(("ax", "ay", "az", "jx", "jy", "jz"), aj))
+    d, extra = _CACHE["ctx"]
+    ip = {f: v[:ni] for f, v in d.items()}
+    jp = {f: v[:nj] for f, v in d.items()}
+    for k in ("phi", "acc", "acc_jerk", "snap_crackle"):
+        if k == "snap_crackle":
+            ip2 = dict(ip, **{f: v[:ni] for f, v in extra.items()})
+            jp2 = dict(jp, **{f: v[:nj] for f, v in extra.items()})
+        else:
+            ip2, jp2 = ip, jp
+        a = kernels.compute(k, ip2, jp2, mode="fp32c")
+        b = kernels.compute(k, ip2, jp2, mode="fp64")
+        if k == "phi":
+            rel = np.abs(a[0] - b[0]) / np.abs(b[0])
+            assert rel.max() < 1e-6, (k, ni, nj, rel.max())
+            continue
+        # per particle: |x32 - x64| / |x64| of each 3-vector output
+        for g in range(len(a) // 3):
+            x = np.stack(a[3 * g:3 * g + 3])
+            y = np.stack(b[3 * g:3 * g + 3])
+            rel = (np.linalg.norm(x - y, axis=0) /
+                   np.linalg.norm(y, axis=0))
+            rms = np.sqrt(np.mean(rel ** 2))
+            if k != "snap_crackle":
+                # acceleration, jerk
+                ok = rel.max() < 1e-6 if g == 0 else (rms < 3e-7 and
+                                                     rel.max() < 3e-5)
+            else:
+                # snap, crackle
+                ok = rms < (1e-6, 1e-5)[g] and rel.max() < (1e-4, 1e-3)[g]
+            assert ok, (k, ni, nj, g, rms, rel.max())
+
+
 def test_newton_third_law_large_n():
     ps = _plummer(32768, 7, 4.0 / 32768)
     d = {f: getattr(ps, f) for f in spec.F8}
