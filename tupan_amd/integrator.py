@@ -596,14 +596,13 @@ class SakuraPN(SIA):
             ps._pair("sakura_near", tau / 2, flag, self.rcut,
                      status=self.status)
             _lib.call_update("apply_delta", ps.n, x, d, st)
-        _lib.call_update("drift", ps.n, r, v, float(tau), 1, None, None, st)
+        # the second half drift, and with PN terms pn_mr += pn_mv * tau
+        # (the whole tau) in the same launch
+        mr = mv = None
         if ps.include_pn_corrections:
-            # pn_mr += pn_mv * tau with r untouched: a drift of the
-            # accumulators alone
-            _lib.call_update(
-                "drift", ps.n, ps.ptrs(("pn_mrx", "pn_mry", "pn_mrz")),
-                ps.ptrs(("pn_mvx", "pn_mvy", "pn_mvz")), float(tau), 0, None,
-                None, st)
+            mr = ps.ptrs(("pn_mrx", "pn_mry", "pn_mrz"))
+            mv = ps.ptrs(("pn_mvx", "pn_mvy", "pn_mvz"))
+        _lib.call_update("drift", ps.n, r, v, float(tau), 1, mr, mv, st)
 
     def decide_tstep(self, ps, tau):
         return tau
