@@ -254,6 +254,14 @@ int tupan_kepler_host(const double *const *ip, double dt, double *const *op);
 int tupan_acc_far_dev(size_t ni, const double *const *ip, size_t nj,
                       const double *const *jp, double rcut, double *const *op,
                       void *stream);
+/* tupan_acc_far_dev of a set on itself (ip = {m, rx, ry, rz, e2}, n each)
+ * followed by tupan_kick_pn_half_dev(v, acc, w, tau) -- kick_pn's first
+ * half kick, tupan/integrator/sia.py:142-156 -- in the same launches: acc
+ * and v end bit-identical to the two calls.  No reference counterpart. */
+int tupan_acc_far_kick_pn_half_dev(size_t n, const double *const *ip,
+                                   double rcut, double *const *acc,
+                                   double *const *v, const double *const *w,
+                                   double tau, void *stream);
 int tupan_sakura_near_dev(size_t ni, const double *const *ip, size_t nj,
                           const double *const *jp, double dt, int flag,
                           double rcut, double *const *op, int *d_status,

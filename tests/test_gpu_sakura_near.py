@@ -10,7 +10,9 @@
   through the near mask and equals the full Sakura kernel (which runs the
   dense loop) bit for bit: the mask selects exactly the contributing pairs
   and keeps their order;
-* a cut-off of zero selects nothing: zero outputs, clean status.
+* a cut-off of zero selects nothing: zero outputs, clean status;
+* acc_far_kick_pn_half (far-pair Acc with kick_pn's first half kick in the
+  same launches) gives the bits of acc_far followed by kick_pn_half.
 """
 import numpy as np
 import pytest
@@ -114,3 +116,31 @@ def test_sakura_near_zero_cut_selects_nothing(torch):
     assert status == 0
     for k in range(6):
         assert not np.any(got[k])
+
+
+@pytest.mark.parametrize("n", [1000, 2048, 70000])
+def test_acc_far_kick_half_is_bitwise_the_two_calls(torch, n):
+    from tupan_amd import _lib
+    rng = np.random.RandomState(n)
+    f5 = [torch.as_tensor(x, device="cuda") for x in (
+        rng.uniform(1, 2, n) / n, rng.standard_normal(n),
+        rng.standard_normal(n), rng.standard_normal(n), np.full(n, 1e-6))]
+    v0 = [torch.as_tensor(rng.standard_normal(n), device="cuda")
+          for _ in range(3)]
+    w = [torch.as_tensor(rng.standard_normal(n), device="cuda")
+         for _ in range(3)]
+    ptr = lambda ts: [t.data_ptr() for t in ts]
+    rcut, tau = 0.05, 1.0 / 1024
+    a1 = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    v1 = [t.clone() for t in v0]
+    _lib.call_update("acc_far", n, ptr(f5), n, ptr(f5), rcut, ptr(a1), None)
+    _lib.call_update("kick_pn_half", n, ptr(v1), ptr(a1), ptr(w), tau, None)
+    a2 = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    v2 = [t.clone() for t in v0]
+    _lib.call_update("acc_far_kick_pn_half", n, ptr(f5), rcut, ptr(a2),
+                     ptr(v2), ptr(w), tau, None)
+    torch.cuda.synchronize()
+    for x, y in zip(a1 + v1, a2 + v2):
+        assert np.array_equal(x.cpu().numpy().view(np.uint64),
+                              y.cpu().numpy().view(np.uint64))
+    assert not np.array_equal(v2[0].cpu().numpy(), v0[0].cpu().numpy())

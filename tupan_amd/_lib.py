@@ -139,6 +139,8 @@ def lib():
         L.tupan_kepler_dev.argtypes = [_PP, _D, _PP, _P, _P]
         L.tupan_kepler_host.argtypes = [_PP, _D, _PP]
         L.tupan_acc_far_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _PP, _P]
+        L.tupan_acc_far_kick_pn_half_dev.argtypes = [_SZ, _PP, _D, _PP, _PP,
+                                                     _PP, _D, _P]
         L.tupan_sakura_near_dev.argtypes = [_SZ, _PP, _SZ, _PP, _D, _I, _D,
                                             _PP, _P, _P]
         _PI = ctypes.POINTER(ctypes.c_int)

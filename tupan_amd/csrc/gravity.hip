@@ -137,6 +137,28 @@ struct AccFarF64 : AccF64 {
   }
 };
 
+// AccFarF64 whose finishing step also applies kick_pn's first half kick,
+// v += (a * tau + w) / 2 (tupan_kick_pn_half_dev's operations, in its
+// order), to the particle whose acceleration it has just written.  The pair
+// terms read masses, positions and softenings only, so updating v where
+// the sums finish (inside the pair kernel when there is one block row)
+// races with no other block.
+struct AccFarKickF64 : AccFarF64 {
+  struct Params : AccFarF64::Params {
+    double tau;
+    double *v[3];
+    const double *w[3];
+  };
+  __device__ static void finish(const double *a, Out out, int i,
+                                const Params &prm) {
+#pragma clang fp contract(off)
+    for (int k = 0; k < 3; ++k) {
+      out.f[k][i] = a[k];
+      prm.v[k][i] = prm.v[k][i] + (a[k] * prm.tau + prm.w[k][i]) / 2;
+    }
+  }
+};
+
 struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
   static constexpr int kFields = 5, kMaxIpl = 4, kPrefIpl = 2;
   using JRec = Pos32cJ;
@@ -390,6 +412,26 @@ int tupan_acc_far_dev(size_t ni, const double *const *ip, size_t nj,
   AccFarF64::Params prm{rcut * rcut};
   return run_engine<AccFarF64>(TUPAN_K_ACC, TUPAN_MODE_FP64, ni, ip, nj, jp,
                                op, prm, as_stream(stream));
+}
+
+int tupan_acc_far_kick_pn_half_dev(size_t n, const double *const *ip,
+                                   double rcut, double *const *acc,
+                                   double *const *v, const double *const *w,
+                                   double tau, void *stream) {
+  if (!(rcut >= 0.0))
+    return set_error(TUPAN_ERR_ARG, "rcut must be >= 0 (got %g)", rcut);
+  if (!v || !w) return set_error(TUPAN_ERR_ARG, "NULL v or w");
+  AccFarKickF64::Params prm{};
+  prm.rc2 = rcut * rcut;
+  prm.tau = tau;
+  for (int k = 0; k < 3; ++k) {
+    if (!v[k] || !w[k])
+      return set_error(TUPAN_ERR_ARG, "NULL v or w field %d", k);
+    prm.v[k] = v[k];
+    prm.w[k] = w[k];
+  }
+  return run_engine<AccFarKickF64>(TUPAN_K_ACC, TUPAN_MODE_FP64, n, ip, n, ip,
+                                   acc, prm, as_stream(stream));
 }
 
 int tupan_acc_jerk_dev(int mode, size_t ni, const double *const *ip,

@@ -472,15 +472,15 @@ class SIA(Base):
         """kick_n, or kick_pn (sia.py:142-156): half kick with w, PN
         accelerations at the half-kicked velocities, the flux bookkeeping
         and the second half kick (the last two fused in one launch)."""
-        self.newton_acc(ps)
         v = ps.ptrs(("vx", "vy", "vz"))
         a = ps.ptrs(("ax", "ay", "az"))
         st = _stream(ps)
         if not ps.include_pn_corrections:
+            self.newton_acc(ps)
             _lib.call_update("kick", ps.n, v, a, float(tau), st)
             return
         w = ps.ptrs(("wx", "wy", "wz"))
-        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
+        self.newton_acc_kick_half(ps, v, a, w, tau, st)
         ps.set_pnacc(self.clight)
         _lib.call_update(
             "pn_flux_kick", ps.n,
@@ -488,6 +488,11 @@ class SIA(Base):
                      "pnax", "pnay", "pnaz")),
             ps.ptrs(("wx", "wy", "wz", "pn_ke", "pn_mvx", "pn_mvy", "pn_mvz",
                      "pn_amx", "pn_amy", "pn_amz")), a, float(tau), st)
+
+    def newton_acc_kick_half(self, ps, v, a, w, tau, st):
+        """Newtonian accelerations, then kick_pn's first half kick."""
+        self.newton_acc(ps)
+        _lib.call_update("kick_pn_half", ps.n, v, a, w, float(tau), st)
 
     def kdk(self, ps, tau):
         self.kick(ps, 0.5 * tau)
@@ -584,6 +589,13 @@ class SakuraPN(SIA):
     def newton_acc(self, ps):
         # the far pairs only (r >= rcut), in FP64 whatever the set's mode
         ps._pair("acc_far", self.rcut, mode="fp64")
+
+    def newton_acc_kick_half(self, ps, v, a, w, tau, st):
+        # the far-pair sums and the first half kick in the same launches
+        # (bit-identical to acc_far followed by kick_pn_half)
+        _lib.call_update("acc_far_kick_pn_half", ps.n,
+                         ps.ptrs(("mass", "rx", "ry", "rz", "eps2")),
+                         float(self.rcut), a, v, w, float(tau), st)
 
     def drift(self, ps, tau):
         r = ps.ptrs(("rx", "ry", "rz"))
