@@ -91,11 +91,28 @@ int tupan_set_launch(int kernel_id, int mode, int ipl, int splits);
  * the recorded launches, returns their summed duration (ms) and count, and
  * clears the record. */
 int tupan_set_timing(int on);
-/* FP32C near field: pairs with r^2 < (m_j/<m>) (c nj^(-1/3))^2 are
- * evaluated in FP64 (a threshold on the pair force, c mean interparticle
- * spacings of a unit-size system for a mean-mass j).  c = 0 turns it off;
- * default 2. */
+/* FP32C near field of the single-system kernels: pairs with
+ * r^2 < (m_j/<m>) (c nj^(-1/3))^2 are evaluated in FP64 (a threshold on
+ * the pair force, c mean interparticle spacings of a unit-size system for
+ * a mean-mass j).  c = 0 leaves only the r^2 = 0 pairs; default 5.  (The
+ * ensemble's FP32C passes use their own factor, 2.) */
 int tupan_set_near_field(double c);
+/* FP32C calls with nj > 4096 run over Morton-sorted copies of the
+ * particles: tiles whose bounding box lies beyond the near field skip the
+ * per-pair near test, and rows whose estimated FP32 error exceeds 0.7 of
+ * the largest measured error-to-estimate ratio (7.1e-7) are recomputed in
+ * FP64.  on = 0 selects the unsorted engine for them (default 1). */
+int tupan_set_sorted(int on);
+/* Sorted FP32C calls recompute in FP64 the rows whose error estimate
+ * sqrt(sum over tiles of |FP32 tile partial|^2) exceeds tau |result| in
+ * some output group; default 0.7 (about 0.5% of the rows of a Plummer
+ * model, every other row within ~6e-7 of FP64).  tau = 0 recomputes every
+ * row with any FP32 tile. */
+int tupan_set_flag_tau(double tau);
+/* Rows of the last sorted FP32C call (on the calling process's last stream
+ * used for one) that were flagged and recomputed in FP64; synchronises
+ * that stream.  0 before any such call. */
+int tupan_last_flagged(int *count);
 /* Sakura: Kepler solves a pair may spend (default 2^20) before it is
  * reported as not converged. */
 int tupan_set_sakura_budget(int solves);

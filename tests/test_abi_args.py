@@ -78,5 +78,14 @@ def test_tuning_knobs_validated(L):
     _err(L, L.tupan_set_launch(99, 0, 0, 0))
     _err(L, L.tupan_set_launch(2, 0, 3, 0))
     _err(L, L.tupan_set_launch(2, 0, 0, -1))
-    assert L.tupan_set_near_field(2.0) == _lib.OK
+    _err(L, L.tupan_set_flag_tau(-0.1))
+    _err(L, L.tupan_set_flag_tau(float("nan")))
+    _err(L, L.tupan_last_flagged(NULL))
+    n = ctypes.c_int(-1)
+    # no sorted call yet in this process: nothing to read, no device needed
+    assert L.tupan_last_flagged(ctypes.byref(n)) == _lib.OK and n.value == 0
+    # restore the defaults
+    assert L.tupan_set_near_field(5.0) == _lib.OK
+    assert L.tupan_set_flag_tau(0.7) == _lib.OK
+    assert L.tupan_set_sorted(1) == _lib.OK
     assert L.tupan_set_sakura_budget(1 << 20) == _lib.OK

@@ -1,6 +1,8 @@
 """bench.py's output contract on a short run: one JSON line on stdout with
 the metric, value and unit, the step counts it was asked for, a roofline
-object whose achieved/peak/frac agree, and a cpu_baseline object."""
+object whose achieved/peak/frac agree, a cpu_baseline object, and the
+metric's accuracy condition (<= 1e-6 per particle against the reference's
+C rows) met by the timed steps."""
 import json
 import os
 import subprocess
@@ -40,3 +42,9 @@ def test_bench_json_line():
         assert k in cb, k
     assert cb["kind"] in ("reference", "port") and cb["cores"] >= 1
     assert cb["fp64_ieee_bitwise_equal"] is True
+    # the metric's accuracy condition, against the reference's C rows
+    acc = r["accuracy"]
+    assert acc["rows"] >= 64
+    assert acc["acc_max"] <= 1e-6 and acc["jerk_max"] <= 1e-6, acc
+    assert r["accuracy_vs_fp64_gpu"]["jerk_max"] <= 1e-6
+    assert r["fp32c_rows_recomputed_fp64"] > 0

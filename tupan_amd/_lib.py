@@ -79,6 +79,9 @@ def lib():
         L.tupan_set_launch.argtypes = [ctypes.c_int] * 4
         L.tupan_set_timing.argtypes = [ctypes.c_int]
         L.tupan_set_near_field.argtypes = [ctypes.c_double]
+        L.tupan_set_sorted.argtypes = [ctypes.c_int]
+        L.tupan_last_flagged.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.tupan_set_flag_tau.argtypes = [ctypes.c_double]
         L.tupan_set_sakura_budget.argtypes = [ctypes.c_int]
         L.tupan_take_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int)]
@@ -182,8 +185,28 @@ def set_launch(kernel, mode, ipl=0, splits=0):
 
 
 def set_near_field(c):
-    """FP32C near-field factor (0 disables the FP64 near-field branch)."""
+    """FP32C near-field factor of the single-system kernels (default 5; 0
+    leaves only the r^2 = 0 pairs to the FP64 branch)."""
     check(lib().tupan_set_near_field(float(c)))
+
+
+def set_flag_tau(tau):
+    """Sorted FP32C: rows whose estimated FP32 error exceeds tau |result|
+    are recomputed in FP64 (default 0.7; 0: every row with FP32 tiles)."""
+    check(lib().tupan_set_flag_tau(float(tau)))
+
+
+def last_flagged():
+    """Rows the last sorted FP32C call recomputed in FP64 (synchronises)."""
+    n = ctypes.c_int(0)
+    check(lib().tupan_last_flagged(ctypes.byref(n)))
+    return n.value
+
+
+def set_sorted(on):
+    """Large FP32C calls (nj > 4096) over Morton-sorted particles with FP64
+    recomputation of flagged rows (default on); off: the unsorted engine."""
+    check(lib().tupan_set_sorted(1 if on else 0))
 
 
 def set_sakura_budget(solves):

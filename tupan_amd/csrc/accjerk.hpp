@@ -48,7 +48,15 @@ struct AccJerkF64 : SumOut<6> {
   }
 };
 
-struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
+// kDebias: the stored mass carries kMassY3 and the velocity offsets
+// kVelY2, which cancel the mean bias of v_rsq_f32 in k = m y^3 and in
+// alpha = 3 (r.v) y^2 at no cost per pair (the single-system kernels);
+// without it, the plain values (the ensemble, which builds its records
+// itself).
+template <bool kDebias>
+struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
+  using Base = F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6>;
+  using typename Base::A32;
   static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
   // Velocity differences from FP32 offsets to the j tile's FP64 velocity
   // origin: the j offset is rounded once in pack, the i offset once per
@@ -63,28 +71,35 @@ struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
     float e2, dvx, dvy, dvz;
     int i;
   };
-  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
-    const double *o = c.origins + 4 * (j / kTile);
+  // record of j with the velocity origin o (3 doubles) of its tile
+  __device__ static JRec pack_o(const Soa &s, int j, const PackCtx &c,
+                                const double *o) {
+    constexpr double vs = kDebias ? kVelY2 : 1.0;
+    constexpr double ms = kDebias ? kMassY3 : 1.0;
     return {s.f[RX][j],
             s.f[RY][j],
             s.f[RZ][j],
-            dsub32(s.f[VX][j], o[0]),
-            dsub32(s.f[VY][j], o[1]),
-            dsub32(s.f[VZ][j], o[2]),
-            (float)s.f[M][j],
+            (float)((s.f[VX][j] - o[0]) * vs),
+            (float)((s.f[VY][j] - o[1]) * vs),
+            (float)((s.f[VZ][j] - o[2]) * vs),
+            (float)(s.f[M][j] * ms),
             (float)s.f[E2][j],
             near_threshold(s.f[M][j], s.f[E2][j], c)};
   }
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
+  __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
+    return pack_o(s, j, c, c.origins + 4 * (j / kTile));
+  }
+  __device__ static I load_i(const Soa &s, int i, const NoParams &) {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i], s.f[VY][i],
             s.f[VZ][i], (float)s.f[E2][i], 0.0f, 0.0f, 0.0f, i};
   }
   // the i-side velocity offsets to tile t's origin
   __device__ static void tile_begin(I &q, const double *origins, int t) {
+    constexpr double vs = kDebias ? kVelY2 : 1.0;
     const double *o = origins + 4 * t;
-    q.dvx = dsub32(q.vx, o[0]);
-    q.dvy = dsub32(q.vy, o[1]);
-    q.dvz = dsub32(q.vz, o[2]);
+    q.dvx = (float)((q.vx - o[0]) * vs);
+    q.dvy = (float)((q.vy - o[1]) * vs);
+    q.dvz = (float)((q.vz - o[2]) * vs);
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
                                 float rz, float s2, bool far, A32 &a) {
@@ -106,5 +121,7 @@ struct AccJerkF32c : F32cBase<AccJerkF32c, AccJerkF64, 6> {
     a.v[5] -= k * vz;
   }
 };
+using AccJerkF32c = AccJerkF32cT<true>;
+using AccJerkF32cTile = AccJerkF32cT<false>;
 
 }  // namespace tupan

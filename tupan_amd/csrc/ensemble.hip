@@ -277,14 +277,19 @@ __global__ __launch_bounds__(kThreads) void ens_pack(int ntot, Fields s,
 
 // ---------------------------------------------------------------- FP32C
 // AccJerk in the FP32C arithmetic of the single-system kernels (f32c.hpp,
-// accjerk.hpp): FP64 separations rounded once to FP32, FP32 tile sums folded
-// into FP64, velocity offsets to per-tile FP64 origins, and near pairs (below
-// the j record's threshold) re-evaluated in FP64 from the slab.  Tiles are
+// accjerk.hpp) without the Newton refinement of 1/sqrt: FP64 separations
+// rounded once to FP32, FP32 tile sums folded into FP64, velocity offsets
+// to per-tile FP64 origins, and near pairs (below the j record's
+// threshold) re-evaluated in FP64 from the slab.  Tiles are
 // numbered within each cluster (tbase: the cluster's first tile), so a work
 // item's j range -- a multiple of the tile length from the cluster's start --
 // covers whole tiles.  The near-field threshold uses each cluster's own
 // factor and largest e2 (ClusterCtx, set with the state).
-using AJ32 = AccJerkF32c;
+using AJ32 = AccJerkF32cTile;
+// Near-field factor of the ensemble's FP32C passes (the single-system
+// kernels' rule with c = 2): clusters of 64-4096 particles in Morton order
+// keep their near pairs to a few per cent of the j's at this radius.
+constexpr double kEnsNearC = 2.0;
 
 struct ClusterCtx {
   double factor, e2max;
@@ -316,7 +321,7 @@ __global__ __launch_bounds__(kThreads) void ens_origins(
   }
 }
 
-// FP32C j records (AccJerkF32c::pack with the cluster's tile origin and
+// FP32C j records (AJ32::pack with the cluster's tile origin and
 // near-field context).
 __global__ __launch_bounds__(kThreads) void ens_pack32(
     int ntot, Fields s, const int *__restrict__ cid,
@@ -636,7 +641,7 @@ struct tupan_ensemble {
   // FP32C AccJerk (mode TUPAN_MODE_FP32C)
   int mode = TUPAN_MODE_FP64;
   int ntiles = 0;
-  AccJerkF32c::JRec *rec32 = nullptr;
+  AJ32::JRec *rec32 = nullptr;
   double *orig = nullptr;
   ClusterCtx *cctx = nullptr;
   int *tcid = nullptr;
@@ -852,7 +857,7 @@ int tupan_ensemble_create(tupan_ensemble **out, int ncl, const int *sizes,
       (er = hipMalloc(&e->pairs, sizeof(double))) != hipSuccess ||
       (er = hipMalloc(&e->count, sizeof(int))) != hipSuccess ||
       (er = hipMalloc(&e->err, sizeof(int))) != hipSuccess ||
-      (er = hipMalloc(&e->rec32, sizeof(AccJerkF32c::JRec) * nt)) !=
+      (er = hipMalloc(&e->rec32, sizeof(AJ32::JRec) * nt)) !=
           hipSuccess ||
       (er = hipMalloc(&e->orig, sizeof(double) * 4 * e->ntiles)) !=
           hipSuccess ||
@@ -966,7 +971,7 @@ int tupan_ensemble_set_state(tupan_ensemble *e, const double *const *fields,
   // FP32C near-field context of each cluster (the single-system rule,
   // ctx_final_kernel: factor = c^2 n^(1/3) / sum m, largest e2)
   std::vector<ClusterCtx> cx(e->ncl);
-  const double cnear = near_field_factor();
+  const double cnear = kEnsNearC;
   for (int c = 0; c < e->ncl; ++c) {
     double msum = 0.0, e2max = 0.0;
     for (int i = e->off[c]; i < e->off[c] + e->n[c]; ++i) {

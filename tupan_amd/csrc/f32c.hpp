@@ -5,7 +5,9 @@
 //     per-tile FP64 origin (policies with kVelOrigins): the i-side offset is
 //     rounded once per tile, so its rounding is shared by 32 pairs only and
 //     averages out over the tiles instead of adding up over all j;
-//   * the rest of the pair arithmetic is FP32 with v_rsq_f32;
+//   * the rest of the pair arithmetic is FP32 with v_rsq_f32, whose mean
+//     bias the single-system policies fold into their stored masses and
+//     velocity offsets (kRsqBias), or refine by a Newton step (rsq32n);
 //   * sums run in FP32 over a tile of kTile j's and are folded into FP64
 //     accumulators after every tile;
 //   * near field: a pair whose softened s^2 is below the j record's
@@ -15,6 +17,31 @@
 #include "engine.hpp"
 
 namespace tupan {
+
+// v_rsq_f32 on MI355X is within 1 ulp but biased low: over every float in
+// [1, 4) (which fixes every argument: rsq(4x) = rsq(x)/2), 9.8% of results
+// are one ulp below the correctly rounded value and 1.3% one ulp above,
+// a mean relative error of -6.98e-9 (log-uniform arguments; -7.39e-9
+// uniform) and an RMS of 2.6e-8 (2.4e-8 when correctly rounded).  That
+// bias enters y^3 of every pair the same way, and it is what limited the
+// FP32 accelerations to ~1e-6.  The single-system policies remove it for
+// free: the stored masses carry (1 + 3 b) (k = m y^3) and the velocity
+// offsets (1 + 2 b) (alpha = 3 (r.v) y^2), rounded once to FP32 with them.
+constexpr double kRsqBias = 6.98e-9;
+constexpr double kMassY3 = 1.0 + 3.0 * kRsqBias;  // masses multiplying y^3
+constexpr double kMassY1 = 1.0 + kRsqBias;        // masses multiplying y
+constexpr double kVelY2 = 1.0 + 2.0 * kRsqBias;   // velocity offsets
+
+// 1/sqrt(x) in FP32 to about 1/2 ulp: v_rsq_f32 refined by one Newton step
+// (SnapCrackle, whose pair terms take y^3..y^9 through the recurrences),
+//   e = 1 - x y^2  (an FMA: the residual keeps its full precision),
+//   y' = y + (y / 2) e.
+// x = 0 gives NaN (0 * inf): callers select 0 for masked pairs afterwards.
+__device__ __forceinline__ float rsq32n(float x) {
+  const float y = rsq32(x);
+  const float e = __builtin_fmaf(-(x * y), y, 1.0f);
+  return __builtin_fmaf(0.5f * y, e, y);
+}
 
 // Threshold on the FP32 s^2 = r^2 + e2_i + e2_j below which a pair is near:
 // (near2_j + e2_j + e2max) widened by 2^-20, so that every pair with r^2 = 0
@@ -33,6 +60,7 @@ template <class Derived, class P64, int N>
 struct F32cBase : SumOut<N> {
   static constexpr bool kTwoLevel = true;
   static constexpr int kVelOrigins = 0;
+  using Exact = P64;  // the FP64 policy (near pairs, recomputed rows)
   using Params = NoParams;
   using A64 = F64Acc<N>;
   using A32 = F32Acc<N>;
@@ -42,6 +70,23 @@ struct F32cBase : SumOut<N> {
   __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   template <class I>
   __device__ static void tile_begin(I &, const double *, int) {}
+
+  // Error estimate of the FP32 tile sums (used by the sorted engine): per
+  // output group -- a 3-vector, or the one scalar output -- the sum over
+  // tiles of |tile partial|^2, accumulated in FP32 after every tile.  The
+  // FP32 error of a row is a random walk whose steps scale with the tile
+  // partials, so sqrt(stat) / |result| predicts it.
+  static constexpr int kGroupLen = N >= 3 ? 3 : 1;
+  static constexpr int kGroups = N / kGroupLen;
+  static constexpr int kPart = N + kGroups;  // partials: outputs, then stats
+  __device__ static void stat(const A32 &a, float *g) {
+#pragma unroll
+    for (int q = 0; q < kGroups; ++q)
+#pragma unroll
+      for (int k = 0; k < kGroupLen; ++k)
+        g[q] = __builtin_fmaf(a.v[q * kGroupLen + k], a.v[q * kGroupLen + k],
+                              g[q]);
+  }
 
   // Separation, softened s^2 and the near-field test.  s^2 uses explicit
   // FMAs so that pair32 and near_pair take bit-identical decisions; a NaN
@@ -64,6 +109,16 @@ struct F32cBase : SumOut<N> {
     const bool far = !near_test(q, r, rx, ry, rz, s2);
     Derived::body32(q, r, rx, ry, rz, s2, far, a);
     return __builtin_amdgcn_ballot_w64(far);
+  }
+  // FP32 contribution of a pair known to be far (a tile whose bounding box
+  // is beyond every near-field threshold of its j's): no near test.
+  template <class I, class JRec>
+  __device__ static void pair32_far(const I &q, const JRec &r, A32 &a) {
+    const float rx = dsub32(q.x, r.x), ry = dsub32(q.y, r.y),
+                rz = dsub32(q.z, r.z);
+    const float s2 = __builtin_fmaf(
+        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+    Derived::body32(q, r, rx, ry, rz, s2, true, a);
   }
   // FP64 contribution of a near pair, from the caller's FP64 arrays.
   template <class I, class JRec>

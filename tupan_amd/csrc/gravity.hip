@@ -5,7 +5,9 @@
 // (f32c.hpp):
 //   * r_ij = x_i - x_j is formed in FP64 and rounded once to FP32
 //     (dsub32), so the separation is accurate to 1/2 ulp of itself;
-//   * the rest of the pair arithmetic is FP32 with v_rsq_f32;
+//   * the rest of the pair arithmetic is FP32 with v_rsq_f32; Phi, Acc and
+//     AccJerk fold its measured mean bias into the stored masses (and
+//     velocity offsets), SnapCrackle refines it by a Newton step;
 //   * sums run in FP32 over a tile of kTile j's and are folded into FP64
 //     accumulators after every tile (block-compensated summation);
 //   * near field: a pair inside its near-field radius (see ctx_kernel)
@@ -60,9 +62,11 @@ struct PhiF64 : SumOut<1> {
 struct Pos32cJ { double x, y, z; float m, e2, near2, pad; };
 struct Pos32cI { double x, y, z; float e2; int i; };
 
+// ms: the mass factor that cancels v_rsq_f32's mean bias (kMassY1 for
+// Phi's m y, kMassY3 for Acc's m y^3)
 __device__ __forceinline__ Pos32cJ pack_pos32c(const Soa &s, int j,
-                                               const PackCtx &c) {
-  return {s.f[RX][j], s.f[RY][j], s.f[RZ][j], (float)s.f[M][j],
+                                               const PackCtx &c, double ms) {
+  return {s.f[RX][j], s.f[RY][j], s.f[RZ][j], (float)(s.f[M][j] * ms),
           (float)s.f[E2][j], near_threshold(s.f[M][j], s.f[E2][j], c), 0.0f};
 }
 __device__ __forceinline__ Pos32cI load_pos32c(const Soa &s, int i) {
@@ -74,7 +78,7 @@ struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
   using JRec = Pos32cJ;
   using I = Pos32cI;
   __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
-    return pack_pos32c(s, j, c);
+    return pack_pos32c(s, j, c, kMassY1);
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return load_pos32c(s, i);
@@ -164,7 +168,7 @@ struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
   using JRec = Pos32cJ;
   using I = Pos32cI;
   __device__ static JRec pack(const Soa &s, int j, const PackCtx &c) {
-    return pack_pos32c(s, j, c);
+    return pack_pos32c(s, j, c, kMassY3);
   }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return load_pos32c(s, i);
@@ -282,7 +286,7 @@ struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
   }
   __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
                                 float rz, float s2, bool far, A32 &a) {
-    float y = rsq32(s2);
+    float y = rsq32n(s2);
     y = far ? y : 0.0f;
     snap_crackle_pair<float>(
         rx, ry, rz, dsub32(q.vx, r.vx), dsub32(q.vy, r.vy),

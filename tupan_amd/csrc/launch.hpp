@@ -26,7 +26,7 @@ void device_free(void *p, hipStream_t st);
 // entry points (no allocation in steady state; safe to capture).  Each
 // buffer in simultaneous use within one call needs its own slot.
 enum ScratchSlot { kSlotRecords = 0, kSlotPartials, kSlotIeee, kSlotReduce,
-                   kSlotPairwise, kSlotMask };
+                   kSlotPairwise, kSlotMask, kSlotSorted };
 
 // Largest near mask (sparse policies) the launcher builds; above it the
 // engine tests the pairs itself.
@@ -55,6 +55,10 @@ int run_ieee_tstep(size_t ni, const double *const *ip, size_t nj,
 
 // ---- generic engine launch ----
 double near_field_factor();
+bool sorted_enabled();
+double flag_tau();
+// the flag word of the last sorted FP32C call (tupan_last_flagged)
+void set_last_flags(const int *flags, hipStream_t st);
 int sakura_budget();
 
 // FP64 policies read their j records straight from the caller's arrays
@@ -88,6 +92,125 @@ template <class Pol>
 struct VelOrigins<Pol, decltype((void)Pol::kVelOrigins)> {
   static constexpr int value = Pol::kVelOrigins;
 };
+
+}  // namespace tupan
+#include "sorted.hpp"
+namespace tupan {
+
+// FP32C over Morton-sorted particles (sorted.hpp), one i per lane.
+template <class Pol>
+int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
+               const Soa &js, const Out &out, hipStream_t st) {
+  using P64 = typename Pol::Exact;
+  bool same = n_i == n_j;
+  for (int k = 0; k < Pol::kFields; ++k) same = same && is.f[k] == js.f[k];
+  const SortGeom gj = sort_geom(n_j), gi = sort_geom(n_i);
+  const int nseg = recompute_chunks(n_j);
+  const SortedLayout L = sorted_layout<Pol>(n_i, n_j, same, gj, gi, nseg);
+  const LaunchShape sh =
+      choose_shape(kernel_id, mode, (size_t)n_i, (size_t)n_j, 1, 1, kTile);
+  int S = sh.splits < kWaves ? kWaves : sh.splits;
+  S = (S + kWaves - 1) / kWaves * kWaves;
+  int jchunk = (n_j + S - 1) / S;
+  jchunk = (jchunk + kTile - 1) / kTile * kTile;
+  const int used = (n_j + jchunk - 1) / jchunk;
+  const int SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
+  const int waves = (n_i + 63) / 64;
+  if (SB > 65535)
+    return set_error(TUPAN_ERR_ARG, "too many j chunks (%d block rows)", SB);
+
+  char *buf = nullptr;
+  double *part = nullptr;
+  int rc = scratch_alloc(kSlotSorted, (void **)&buf, L.total, st);
+  if (rc) return rc;
+  rc = scratch_alloc(kSlotPartials, (void **)&part,
+                     sizeof(double) * (size_t)SB * Pol::kPart * (size_t)n_i,
+                     st);
+  if (rc) {
+    scratch_free(kSlotSorted, buf, st);
+    return rc;
+  }
+  auto at = [&](size_t off) { return (void *)(buf + off); };
+  double *ctxpart = (double *)at(L.ctxpart);
+  double *ctx = (double *)at(L.ctx);
+  int *flags = (int *)at(L.flags);
+  SoaW sjw{}, siw{};
+  Soa sjr{}, sir{};
+  for (int k = 0; k < Pol::kFields; ++k) {
+    sjw.f[k] = (double *)at(L.sj) + (size_t)k * n_j;
+    siw.f[k] = (double *)at(L.si) + (size_t)k * n_i;
+    sjr.f[k] = sjw.f[k];
+    sir.f[k] = siw.f[k];
+  }
+  hipLaunchKernelGGL(sctx_partial_kernel, dim3(kCtxBlocks), dim3(256), 0, st,
+                     js, n_j, is, n_i, ctxpart, (int *)at(L.zero),
+                     (int)L.nzero);
+  // one LSD radix sort per set: keys and indices ping-pong 0 -> 1 -> 0 -> 1
+  auto sort_set = [&](int set, const Soa &f, int n, int box, SortGeom g,
+                      double *ctx_out) -> const int * {
+    uint32_t *kb[2] = {(uint32_t *)at(L.keys[set][0]),
+                       (uint32_t *)at(L.keys[set][1])};
+    int *vb[2] = {(int *)at(L.vals[set][0]), (int *)at(L.vals[set][1])};
+    int *hist[kSortPasses];
+    for (int q = 0; q < kSortPasses; ++q) hist[q] = (int *)at(L.hist[set][q]);
+    hipLaunchKernelGGL(morton_key_kernel, dim3(g.nb), dim3(256), 0, st,
+                       f.f[RX], f.f[RY], f.f[RZ], n, (const double *)ctxpart,
+                       box, g, kb[0], vb[0], hist[0], ctx_out, n_j,
+                       near_field_factor());
+    int dest_bits = 0;
+    while ((1 << dest_bits) < g.nb) ++dest_bits;
+    for (int q = 0; q < kSortPasses; ++q)
+      hipLaunchKernelGGL(radix_scatter_kernel, dim3(g.nb), dim3(256), 0, st,
+                         (const uint32_t *)kb[q & 1], (const int *)vb[q & 1],
+                         kb[(q + 1) & 1], vb[(q + 1) & 1], n, g,
+                         (const int *)hist[q], q * kSortDigit,
+                         q + 1 < kSortPasses ? hist[q + 1] : nullptr,
+                         dest_bits);
+    return vb[kSortPasses & 1];
+  };
+  const int *perm_j = sort_set(0, js, n_j, 2, gj, ctx);
+  const int *perm_i = same ? perm_j : sort_set(1, is, n_i, 8, gi, nullptr);
+  double *origins = (double *)at(L.origins);
+  const TileBox *boxes = (const TileBox *)at(L.boxes);
+  auto *recs = (typename Pol::JRec *)at(L.recs);
+  hipLaunchKernelGGL((sorted_pack_kernel<Pol>), dim3((n_j + 255) / 256),
+                     dim3(256), 0, st, js, n_j, perm_j, sjw,
+                     (const double *)ctx, recs, origins, (TileBox *)boxes);
+  if (!same)
+    hipLaunchKernelGGL((sorted_gather_kernel<Pol::kFields>),
+                       dim3((n_i + 255) / 256), dim3(256), 0, st, is, n_i,
+                       perm_i, siw);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const bool timed = timing_begin(&ev0, &ev1, st);
+  if (timed) (void)hipEventRecord(ev0, st);
+  hipLaunchKernelGGL((sorted_engine<Pol>), dim3((unsigned)waves, (unsigned)SB),
+                     dim3(kBlock), 0, st, sir, n_i, sjr,
+                     (const typename Pol::JRec *)recs, n_j, boxes, jchunk, part,
+                     (const double *)origins);
+  if (timed) (void)hipEventRecord(ev1, st);
+  hipLaunchKernelGGL((sorted_finish_kernel<Pol>), dim3((n_i + 255) / 256),
+                     dim3(256), 0, st, (const double *)part, SB, n_i, perm_i,
+                     out, flag_tau() * flag_tau(), flags);
+  double *rpart = (double *)at(L.rpart);
+  {
+    const int cw = (n_j + nseg - 1) / nseg;
+    const int items = (kRecomputeRows / 64) * nseg;
+    hipLaunchKernelGGL((sorted_recompute_kernel<P64>),
+                       dim3((items + kWaves - 1) / kWaves), dim3(kBlock), 0, st,
+                       sir, sjr, n_j, (const int *)flags, cw, nseg,
+                       kRecomputeRows, rpart);
+    hipLaunchKernelGGL((sorted_recompute_finish<P64>), dim3(kRecomputeBlocks),
+                       dim3(256), 0, st, sir, sjr, n_j, (const int *)flags,
+                       nseg, kRecomputeRows, (const double *)rpart, perm_i,
+                       out);
+    set_last_flags(flags, st);
+  }
+  const hipError_t e = hipGetLastError();
+  scratch_free(kSlotPartials, part, st);
+  scratch_free(kSlotSorted, buf, st);
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return TUPAN_OK;
+}
 
 // FP32C calls with at most this many j's (and kSmallCtxI i's) build their
 // context, tile origins and records in one single-block launch
@@ -171,6 +294,11 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
     out.f[k] = op[k];
   }
   const int n_i = (int)ni, n_j = (int)nj;
+  if constexpr (Pol::kTwoLevel) {
+    // large FP32C calls: Morton-sorted particles (sorted.hpp)
+    if (n_j > kSmallCtxJ && sorted_enabled())
+      return run_sorted<Pol>(kernel_id, mode, n_i, is, n_j, js, out, st);
+  }
   // j chunks are whole tiles for the tiled loops (FP32C, sparse policies);
   // the plain FP64 loop takes chunks of 8 records, which tiny systems use
   constexpr int kChunk =

@@ -31,12 +31,32 @@ int hip_fail(hipError_t e, const char *what) {
 static std::mutex g_mu;
 static LaunchShape g_override[TUPAN_K_COUNT][TUPAN_MODE_COUNT];  // zero = automatic
 static int g_default_mode = TUPAN_MODE_FP64;
-static double g_near_c = 2.0;          // near-field radius, see engine.hpp
+static double g_near_c = 5.0;          // near-field radius, see engine.hpp
+static double g_flag_tau = kFlagTau;   // sorted FP32C: rows recomputed in FP64
+static bool g_sorted = true;           // large FP32C calls in Morton order
 static int g_sakura_budget = 1 << 20;  // Kepler solves per Sakura pair
 
 double near_field_factor() {
   std::lock_guard<std::mutex> lk(g_mu);
   return g_near_c;
+}
+
+double flag_tau() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_flag_tau;
+}
+
+bool sorted_enabled() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_sorted;
+}
+
+static const int *g_last_flags = nullptr;
+static hipStream_t g_last_flags_stream = nullptr;
+void set_last_flags(const int *flags, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_last_flags = flags;
+  g_last_flags_stream = st;
 }
 
 int sakura_budget() {
@@ -307,6 +327,37 @@ int tupan_set_near_field(double c) {
     return set_error(TUPAN_ERR_ARG, "near-field factor must be in [0, 1e6]");
   std::lock_guard<std::mutex> lk(g_mu);
   g_near_c = c;
+  return TUPAN_OK;
+}
+
+int tupan_set_sorted(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_sorted = on != 0;
+  return TUPAN_OK;
+}
+
+int tupan_set_flag_tau(double tau) {
+  if (!(tau >= 0.0) || tau > 1e6)
+    return set_error(TUPAN_ERR_ARG, "flag threshold must be in [0, 1e6]");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_flag_tau = tau;
+  return TUPAN_OK;
+}
+
+int tupan_last_flagged(int *count) {
+  if (!count) return set_error(TUPAN_ERR_ARG, "NULL count");
+  const int *f;
+  hipStream_t st;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    f = g_last_flags;
+    st = g_last_flags_stream;
+  }
+  *count = 0;
+  if (!f) return TUPAN_OK;
+  hipError_t e = hipMemcpyAsync(count, f, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "reading the flag count");
   return TUPAN_OK;
 }
 

@@ -1,0 +1,750 @@
+// sorted.hpp -- the FP32C pair sums of large calls over Morton-sorted
+// particles.
+//
+// Why: FP32 pair arithmetic meets 1e-6 per particle only when the largest
+// pair terms of every particle are summed in FP64 (a near field of several
+// mean interparticle spacings) and the rare rows whose sums cancel badly
+// are recomputed in FP64.  With the particles in input order the near
+// pairs of a wave's 64 i's touch most j's, and each costs an FP64 pair for
+// the whole wave; in Morton order a wave's i's and a tile's j's are
+// compact, the union of a wave's near j's stays ~1% of nj, and a tile
+// whose bounding box is beyond every near-field threshold of its j's needs
+// no per-pair near test at all.
+//
+// One call (ni i's, nj j's; i and j sorted separately unless they are the
+// same arrays):
+//   sctx_partial_kernel   per-block sums/extrema: sum m_j, max e2_i, the
+//                         position boxes; clears the radix histograms of
+//                         passes 1..2 and the flagged-row count
+//   morton_key_kernel     30-bit Morton key (10 bits per axis over the
+//                         set's box), histogram of the first 10-bit digit;
+//                         block 0 writes the near-field context
+//   radix_scatter_kernel  x3: stable LSD radix passes of 10 bits; each
+//                         block sums its digit offsets from the per-block
+//                         histograms, ranks its keys with wave ballots and
+//                         counts the next digit into the next pass's
+//                         histogram -> the permutation (sorted position ->
+//                         caller's index)
+//   sorted_pack_kernel    sorted FP64 copies of the j fields, per-tile
+//                         velocity origins and bounding boxes (with the
+//                         tile's largest near-field threshold), j records
+//   (sorted_gather_kernel the i fields, when i and j differ)
+//   sorted_engine         pair_engine's FP32C loop over sorted particles:
+//                         per 64 tiles, one lane per tile tests the tile's
+//                         box against the wave's box; far tiles run the
+//                         pair loop without the near test, near tiles the
+//                         tested loop with FP64 re-evaluation of near pairs;
+//                         every tile's FP32 partial also feeds the error
+//                         estimate (F32cBase::stat)
+//   sorted_finish_kernel  adds the block-row partials in row order, writes
+//                         each output to the caller's index and flags the
+//                         rows whose estimated FP32 error exceeds
+//                         kFlagTau * |result| (tau^2 on squares)
+//   sorted_recompute_kernel + sorted_recompute_finish
+//                         the flagged rows again in FP64 (the FP64 policy):
+//                         64 rows per wave over j chunks, then the chunk
+//                         partials of each row added in a fixed order:
+//                         bit-reproducible whatever order the rows were
+//                         flagged in
+// No memset anywhere (the clears are kernels), so a call can be captured.
+#pragma once
+#include "f32c.hpp"
+
+namespace tupan {
+
+constexpr int kSortDigit = 10;                 // bits per radix pass
+constexpr int kSortBins = 1 << kSortDigit;
+constexpr int kSortPasses = 3;                 // 30-bit keys
+constexpr int kSortChunk = 2048;               // keys per block (minimum)
+constexpr int kSortMaxBlocks = 256;
+constexpr int kSctxVals = 14;                  // per-block context values
+// Rows whose estimated FP32 error exceeds this fraction of |result| are
+// recomputed in FP64 (the default of tupan_set_flag_tau).  The estimate
+// sqrt(sum_t |tile partial|^2) bounded the error of every row by 8.7e-7 of
+// itself (CPU emulation of these kernels' arithmetic, bit for bit with the
+// GPU's, on the Plummer models N = 16384, 65536 and 262144, seed 1), so an
+// unflagged row stays below 0.7 * 8.7e-7 = 6.1e-7 (3.7e-7 measured); 0.3-0.6%
+// of the rows are flagged at the default near field (c = 5).
+constexpr double kFlagTau = 0.7;
+constexpr int kRecomputeRows = 1024;  // flagged rows recomputed in chunks
+constexpr int kRecomputeMinJ = 64;    // j's per recompute chunk (minimum)
+constexpr int kRecomputeChunks = 1024;
+constexpr int kRecomputeBlocks = 1024;  // sorted_recompute_finish
+
+// Chunks of the FP64 recomputation: at most kRecomputeChunks of at least
+// kRecomputeMinJ j's, so that the few row-waves still fill the chip.
+inline int recompute_chunks(int nj) {
+  int n = (nj + kRecomputeMinJ - 1) / kRecomputeMinJ;
+  return n < 1 ? 1 : (n > kRecomputeChunks ? kRecomputeChunks : n);
+}
+
+// Geometry of one radix sort: nb blocks of ch keys (ch a multiple of 256).
+struct SortGeom {
+  int nb, ch;
+};
+inline SortGeom sort_geom(int n) {
+  int nb = (n + kSortChunk - 1) / kSortChunk;
+  if (nb > kSortMaxBlocks) nb = kSortMaxBlocks;
+  if (nb < 1) nb = 1;
+  int ch = (n + nb - 1) / nb;
+  ch = (ch + 255) / 256 * 256;
+  return SortGeom{nb, ch};
+}
+
+struct TileBox {
+  double lo[3], hi[3];
+  double near2;  // largest near-field threshold of the tile's j records
+  double pad;
+};
+
+struct SoaW {
+  double *f[14];
+};
+
+// a value equal in every lane, moved to scalar registers
+__device__ __forceinline__ double uniform(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return uniform(v);
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return uniform(v);
+}
+// min/max over the 32 lanes of a half wave (one tile)
+__device__ __forceinline__ double tile_min(double v) {
+#pragma unroll
+  for (int m = 16; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m, 32));
+  return v;
+}
+__device__ __forceinline__ double tile_max(double v) {
+#pragma unroll
+  for (int m = 16; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m, 32));
+  return v;
+}
+
+// Per block b: [0] sum m_j, [1] max e2_i, [2..7] j box (lo xyz, hi xyz),
+// [8..13] i box.  Grid kCtxBlocks x 256.  Also clears `nzero` ints at
+// `zero` (histograms of passes 1..2, the flag count).
+static __global__ __launch_bounds__(256) void sctx_partial_kernel(
+    Soa js, int nj, Soa is, int ni, double *__restrict__ part,
+    int *__restrict__ zero, int nzero) {
+  __shared__ double red[kSctxVals][256];
+  const int t = threadIdx.x;
+  double v[kSctxVals];
+  v[0] = 0.0;
+  v[1] = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    v[2 + k] = v[8 + k] = INFINITY;
+    v[5 + k] = v[11 + k] = -INFINITY;
+  }
+  const int stride = kCtxBlocks * 256;
+  for (int j = blockIdx.x * 256 + t; j < nj; j += stride) {
+    v[0] += js.f[M][j];
+    for (int k = 0; k < 3; ++k) {
+      const double x = js.f[RX + k][j];
+      v[2 + k] = fmin(v[2 + k], x);
+      v[5 + k] = fmax(v[5 + k], x);
+    }
+  }
+  for (int i = blockIdx.x * 256 + t; i < ni; i += stride) {
+    v[1] = fmax(v[1], is.f[E2][i]);
+    for (int k = 0; k < 3; ++k) {
+      const double x = is.f[RX + k][i];
+      v[8 + k] = fmin(v[8 + k], x);
+      v[11 + k] = fmax(v[11 + k], x);
+    }
+  }
+  for (int k = 0; k < kSctxVals; ++k) red[k][t] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      red[0][t] += red[0][t + w];
+      red[1][t] = fmax(red[1][t], red[1][t + w]);
+      for (int k = 0; k < 3; ++k) {
+        red[2 + k][t] = fmin(red[2 + k][t], red[2 + k][t + w]);
+        red[5 + k][t] = fmax(red[5 + k][t], red[5 + k][t + w]);
+        red[8 + k][t] = fmin(red[8 + k][t], red[8 + k][t + w]);
+        red[11 + k][t] = fmax(red[11 + k][t], red[11 + k][t + w]);
+      }
+    }
+    __syncthreads();
+  }
+  if (t < kSctxVals) part[blockIdx.x * kSctxVals + t] = red[t][0];
+  for (int q = blockIdx.x * 256 + t; q < nzero; q += stride) zero[q] = 0;
+}
+
+// Lanes of `live` whose value of the low `bits` bits of d equals this
+// lane's (one ballot per bit).
+__device__ __forceinline__ uint64_t match_bits(uint32_t d, int bits,
+                                               uint64_t live) {
+  uint64_t same = live;
+  for (int b = 0; b < bits; ++b) {
+    const bool on = (d >> b) & 1u;
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(on);
+    same &= on ? bal : ~bal;
+  }
+  return same;
+}
+__device__ __forceinline__ uint64_t match_digit(int d, uint64_t live) {
+  return match_bits((uint32_t)d, kSortDigit, live);
+}
+// number of lanes of `same` below this lane
+__device__ __forceinline__ int lane_rank(uint64_t same) {
+  return (int)__builtin_amdgcn_mbcnt_hi(
+      (uint32_t)(same >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+}
+
+// bits 0..9 of v spread to bits 0, 3, 6, ..., 27
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// Morton keys of one set and the histogram of their first digit, one block
+// per chunk of ch keys: hist[d * nb + b].  box: offset of the set's box in
+// the per-block context values.  Block 0 also writes ctx[0] (near-field
+// factor, ctx_final_kernel's rule) and ctx[1] (largest e2_i) when ctx is
+// given.
+static __global__ __launch_bounds__(256) void morton_key_kernel(
+    const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, int n, const double *__restrict__ part,
+    int box, SortGeom g, uint32_t *__restrict__ keys, int *__restrict__ vals,
+    int *__restrict__ hist, double *__restrict__ ctx, int nj, double c) {
+  __shared__ int h[kSortBins];
+  __shared__ double bx[6][kCtxBlocks];
+  const int t = threadIdx.x;
+  for (int d = t; d < kSortBins; d += 256) h[d] = 0;
+  // the set's box from the kCtxBlocks partials (thread b: partial b)
+  if (t < kCtxBlocks)
+    for (int k = 0; k < 6; ++k) bx[k][t] = part[t * kSctxVals + box + k];
+  __syncthreads();
+  for (int w = kCtxBlocks / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int k = 0; k < 3; ++k) {
+        bx[k][t] = fmin(bx[k][t], bx[k][t + w]);
+        bx[3 + k][t] = fmax(bx[3 + k][t], bx[3 + k][t + w]);
+      }
+    __syncthreads();
+  }
+  double lo[3], inv[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = bx[k][0];
+    inv[k] = bx[3 + k][0] > lo[k] ? 1024.0 / (bx[3 + k][0] - lo[k]) : 0.0;
+  }
+  if (ctx && blockIdx.x == 0 && t == 0) {
+    double msum = 0.0, e2max = 0.0;
+    for (int b = 0; b < kCtxBlocks; ++b) {
+      msum += part[b * kSctxVals];
+      e2max = fmax(e2max, part[b * kSctxVals + 1]);
+    }
+    const double f = c * c * cbrt((double)nj) / msum;
+    ctx[0] = (c > 0.0 && msum > 0.0 && f < 1e300) ? f : 0.0;
+    ctx[1] = e2max;
+  }
+  const double *f[3] = {x, y, z};
+  const int e0 = blockIdx.x * g.ch;
+  for (int r = 0; r < g.ch; r += 256) {
+    const int e = e0 + r + t;
+    const bool valid = e < n;
+    uint32_t key = 0;
+    if (valid) {
+      for (int k = 0; k < 3; ++k) {
+        const double q = (f[k][e] - lo[k]) * inv[k];
+        // NaN and the low edge -> 0, the high edge -> 1023
+        const uint32_t b = q >= 1023.0 ? 1023u : (q > 0.0 ? (uint32_t)q : 0u);
+        key |= spread10(b) << k;
+      }
+      keys[e] = key;
+      vals[e] = e;
+    }
+    const int d = (int)(key & (kSortBins - 1));
+    const uint64_t same = match_digit(d, __builtin_amdgcn_ballot_w64(valid));
+    if (valid && lane_rank(same) == 0)
+      atomicAdd(&h[d], __builtin_popcountll(same));
+  }
+  __syncthreads();
+  for (int d = t; d < kSortBins; d += 256) hist[blockIdx.x * kSortBins + d] = h[d];
+}
+
+// One stable LSD pass on digit (key >> shift) & 1023, one block per chunk
+// of ch keys.  Digit d of block b starts at (keys of smaller digits) +
+// (keys of digit d in blocks before b), both summed here from the
+// block-major per-block histograms hist[c * 1024 + d] (coalesced loads);
+// within the block, rounds of 256 keys in index order, each wave ranking
+// its keys of equal digit with ballots (mbcnt = rank among them) and the
+// waves taking their positions in wave order.  hist_next (optional) gets
+// the next digit's per-block counts of the output order: lanes of a wave
+// with the same (next digit, output block) add their count with one
+// atomic.
+static __global__ __launch_bounds__(256) void radix_scatter_kernel(
+    const uint32_t *__restrict__ kin, const int *__restrict__ vin,
+    uint32_t *__restrict__ kout, int *__restrict__ vout, int n, SortGeom g,
+    const int *__restrict__ hist, int shift, int *__restrict__ hist_next,
+    int dest_bits) {
+  __shared__ int base[kSortBins];
+  __shared__ int tot[kSortBins];
+  __shared__ int tsum[256];
+  const int t = threadIdx.x;
+  const int b = blockIdx.x;
+  {
+    int all[4] = {0, 0, 0, 0}, pre[4] = {0, 0, 0, 0};
+#pragma unroll 8
+    for (int c = 0; c < g.nb; ++c) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int v = hist[c * kSortBins + t + 256 * q];
+        all[q] += v;
+        pre[q] += c < b ? v : 0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      tot[t + 256 * q] = all[q];
+      base[t + 256 * q] = pre[q];
+    }
+  }
+  __syncthreads();
+  // exclusive scan of tot over the digits: 4 consecutive per thread, then
+  // the 256 thread sums (Hillis-Steele)
+  int run = 0, loc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    loc[q] = run;
+    run += tot[4 * t + q];
+  }
+  tsum[t] = run;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int v = t >= o ? tsum[t - o] : 0;
+    __syncthreads();
+    tsum[t] += v;
+    __syncthreads();
+  }
+  const int excl = t > 0 ? tsum[t - 1] : 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) base[4 * t + q] += excl + loc[q];
+  __syncthreads();
+
+  const int wave = t >> 6;
+  const int e0 = b * g.ch;
+  // keys of the next round are loaded while this one is placed
+  int e = e0 + t;
+  uint32_t key = e < n ? kin[e] : 0u;
+  int val = e < n ? vin[e] : 0;
+  for (int r = 0; r < g.ch; r += 256) {
+    const bool valid = e < n;
+    const int en = e + 256;
+    const bool nvalid = r + 256 < g.ch && en < n;
+    const uint32_t nkey = nvalid ? kin[en] : 0u;
+    const int nval = nvalid ? vin[en] : 0;
+    const int d = (int)((key >> shift) & (kSortBins - 1));
+    const uint64_t same = match_digit(d, __builtin_amdgcn_ballot_w64(valid));
+    const int rank = lane_rank(same);
+    const int cnt = __builtin_popcountll(same);
+    int pos = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      if (wave == w && valid) {
+        const int bse = base[d];
+        pos = bse + rank;
+        // the group's lowest lane advances the digit; its store depends on
+        // the value the whole wave has just read
+        if (rank == 0) base[d] = bse + cnt;
+      }
+      __syncthreads();
+    }
+    if (valid) {
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    if (hist_next) {
+      const int dn = (int)((key >> (shift + kSortDigit)) & (kSortBins - 1));
+      const int dest = pos / g.ch;
+      const uint64_t grp =
+          match_bits((uint32_t)dest, dest_bits,
+                     match_digit(dn, __builtin_amdgcn_ballot_w64(valid)));
+      if (valid && lane_rank(grp) == 0)
+        atomicAdd(&hist_next[(size_t)dest * kSortBins + dn],
+                  __builtin_popcountll(grp));
+    }
+    e = en;
+    key = nkey;
+    val = nval;
+  }
+}
+
+// Sorted FP64 copies of the j fields, per-tile origins and boxes, and the
+// j records, one thread per sorted position p; a tile is the 32 lanes of a
+// half wave.  The last tile's spare lanes repeat its last j, which changes
+// no minimum or maximum.
+template <class Pol>
+__global__ __launch_bounds__(256) void sorted_pack_kernel(
+    Soa js, int nj, const int *__restrict__ perm, SoaW sj,
+    const double *__restrict__ ctx, typename Pol::JRec *__restrict__ jr,
+    double *__restrict__ origins, TileBox *__restrict__ boxes) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int o = perm[min(p, nj - 1)];
+  if (p < nj)
+    for (int k = 0; k < Pol::kFields; ++k) sj.f[k][p] = js.f[k][o];
+  double org[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (VelOrigins<Pol>::value) {
+    for (int k = 0; k < 3; ++k) {
+      const double v = js.f[VX + k][o];
+      org[k] = 0.5 * (tile_min(v) + tile_max(v));
+    }
+  }
+  const PackCtx pc{ctx[0], ctx[1], nullptr};
+  typename Pol::JRec rec;
+  if constexpr (VelOrigins<Pol>::value)
+    rec = Pol::pack_o(js, o, pc, org);
+  else
+    rec = Pol::pack(js, o, pc);
+  if (p < nj) jr[p] = rec;
+  TileBox bx;
+  for (int k = 0; k < 3; ++k) {
+    const double x = js.f[RX + k][o];
+    bx.lo[k] = tile_min(x);
+    bx.hi[k] = tile_max(x);
+  }
+  bx.near2 = tile_max((double)rec.near2);
+  bx.pad = 0.0;
+  if ((threadIdx.x & (kTile - 1)) == 0 && p < nj) {
+    const int t = p / kTile;
+    for (int k = 0; k < 4; ++k) origins[4 * t + k] = org[k];
+    boxes[t] = bx;
+  }
+}
+
+// Sorted FP64 copies of the i fields (i and j differ).
+template <int NF>
+__global__ __launch_bounds__(256) void sorted_gather_kernel(
+    Soa is, int ni, const int *__restrict__ perm, SoaW si) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= ni) return;
+  const int o = perm[p];
+  for (int k = 0; k < NF; ++k) si.f[k][p] = is.f[k][o];
+}
+
+// pair_engine's FP32C path (kTwoLevel policies, one i per lane) over
+// sorted particles.  Always writes its block-row partials (outputs, then
+// the error statistics) for sorted_finish_kernel.
+template <class Pol>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void sorted_engine(
+    Soa si, int ni, Soa sj, const typename Pol::JRec *__restrict__ jr, int nj,
+    const TileBox *__restrict__ boxes, int jchunk,
+    double *__restrict__ part, const double *__restrict__ aux) {
+  constexpr int kPart = Pol::kPart;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int row = blockIdx.y;
+  const int i0 = blockIdx.x * 64;
+  if (i0 >= ni) return;  // block-uniform
+  const int s = row * kWaves + wave;
+  const int i = min(i0 + lane, ni - 1);
+  const typename Pol::Params prm{};
+  typename Pol::I q = Pol::load_i(si, i, prm);
+  typename Pol::A64 acc;
+  Pol::zero(acc);
+  float sig[Pol::kGroups];
+#pragma unroll
+  for (int g = 0; g < Pol::kGroups; ++g) sig[g] = 0.0f;
+  // the wave's box (spare lanes repeat the last i)
+  double wlo[3], whi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double x = si.f[RX + k][i];
+    wlo[k] = wave_min(x);
+    whi[k] = wave_max(x);
+  }
+
+  const int jb = s * jchunk;
+  const int je = min(nj, jb + jchunk);
+  uint64_t farbits = 0;
+  int tt = 0;
+  for (int j0 = jb; j0 < je; j0 += kTile, ++tt) {
+    if ((tt & 63) == 0) {
+      // lane l tests tile j0 / kTile + l: far when the squared gap between
+      // the boxes reaches the tile's largest near threshold (then every
+      // pair has s^2 >= r^2 >= gap^2 >= near2_j, and r^2 > 0)
+      const int t = j0 / kTile + lane;
+      bool far = false;
+      if (j0 + lane * kTile < je) {
+        const TileBox b = boxes[t];
+        double d2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double gap = fmax(fmax(b.lo[k] - whi[k], wlo[k] - b.hi[k]), 0.0);
+          d2 = __builtin_fma(gap, gap, d2);
+        }
+        far = d2 >= b.near2;
+      }
+      farbits = __builtin_amdgcn_ballot_w64(far);
+    }
+    const int nk = min(kTile, je - j0);
+    typename Pol::A32 a;
+    Pol::zero(a);
+    Pol::tile_begin(q, aux, j0 / kTile);
+    typename Pol::JRec r = jr[j0];
+    if ((farbits >> (tt & 63)) & 1) {
+      if (nk == kTile) {
+#pragma unroll 2
+        for (int k = 0; k < kTile; ++k) {
+          const typename Pol::JRec rn = jr[j0 + min(k + 1, kTile - 1)];
+          Pol::pair32_far(q, r, a);
+          r = rn;
+        }
+      } else {
+        for (int k = 0; k < nk; ++k) {
+          const typename Pol::JRec rn = jr[j0 + min(k + 1, nk - 1)];
+          Pol::pair32_far(q, r, a);
+          r = rn;
+        }
+      }
+      Pol::fold(a, acc);
+      Pol::stat(a, sig);
+    } else {
+      uint32_t near = 0u;
+      if (nk == kTile) {
+#pragma unroll 2
+        for (int k = 0; k < kTile; ++k) {
+          const typename Pol::JRec rn = jr[j0 + min(k + 1, kTile - 1)];
+          near |= near_bit(Pol::pair32(q, r, a)) << k;
+          r = rn;
+        }
+      } else {
+        for (int k = 0; k < nk; ++k) {
+          const typename Pol::JRec rn = jr[j0 + min(k + 1, nk - 1)];
+          near |= near_bit(Pol::pair32(q, r, a)) << k;
+          r = rn;
+        }
+      }
+      Pol::fold(a, acc);
+      Pol::stat(a, sig);
+      while (near) {
+        const int k = __builtin_ctz(near);
+        near &= near - 1u;
+        Pol::near_pair(q, jr[j0 + k], acc, si, sj, j0 + k);
+      }
+    }
+  }
+
+  double v[kPart];
+  Pol::values(acc, v);
+#pragma unroll
+  for (int g = 0; g < Pol::kGroups; ++g) v[Pol::kOut + g] = (double)sig[g];
+  __shared__ double red[kWaves - 1][kPart][64];
+  if (wave > 0) {
+#pragma unroll
+    for (int k = 0; k < kPart; ++k) red[wave - 1][k][lane] = v[k];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  for (int w = 0; w < kWaves - 1; ++w) {
+#pragma unroll
+    for (int k = 0; k < kPart; ++k) v[k] += red[w][k][lane];
+  }
+  if (i0 + lane < ni) {
+#pragma unroll
+    for (int k = 0; k < kPart; ++k)
+      part[((size_t)row * kPart + k) * ni + i0 + lane] = v[k];
+  }
+}
+
+// Row partials -> outputs at the caller's index; rows whose error estimate
+// exceeds tau^2 |result|^2 in some output group go to the flag list
+// (flags[0]: count, flags[1..]: sorted row positions).
+template <class Pol>
+__global__ __launch_bounds__(256) void sorted_finish_kernel(
+    const double *__restrict__ part, int S, int ni,
+    const int *__restrict__ perm_i, Out out, double tau2,
+    int *__restrict__ flags) {
+  constexpr int kPart = Pol::kPart;
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= ni) return;
+  double v[kPart];
+#pragma unroll
+  for (int k = 0; k < kPart; ++k) v[k] = part[(size_t)k * ni + p];
+  for (int r = 1; r < S; ++r) {
+#pragma unroll
+    for (int k = 0; k < kPart; ++k) v[k] += part[((size_t)r * kPart + k) * ni + p];
+  }
+  const int o = perm_i[p];
+#pragma unroll
+  for (int k = 0; k < Pol::kOut; ++k) out.f[k][o] = v[k];
+  bool flag = false;
+#pragma unroll
+  for (int g = 0; g < Pol::kGroups; ++g) {
+    double n2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < Pol::kGroupLen; ++k)
+      n2 += v[g * Pol::kGroupLen + k] * v[g * Pol::kGroupLen + k];
+    flag = flag || v[Pol::kOut + g] > tau2 * n2;
+  }
+  if (flag) flags[1 + atomicAdd(flags, 1)] = p;
+}
+
+// The flagged rows in FP64 (P64, the FP64 policy of the same kernel), in
+// the pair engine's shape: a wave holds 64 flagged rows (one per lane) and
+// walks one chunk of cw j's, so the j data is read once per 64 rows.  The
+// chunk passes through LDS 64 records at a time (one coalesced load per
+// lane, then wave-uniform LDS reads), so a wave waits for memory once per
+// 64 pairs rather than once per pair.  Work item u = (row-wave u / nchunks,
+// chunk u % nchunks) for the first min(count, cap) flagged rows; waves
+// without rows still take part in the block's barriers.  Partials go to
+// rpart[(chunk * cap + row) * kOut + k].
+template <class P64>
+__global__ __launch_bounds__(kBlock) void sorted_recompute_kernel(
+    Soa si, Soa sj, int nj, const int *__restrict__ flags, int cw,
+    int nchunks, int cap, double *__restrict__ rpart) {
+  constexpr int K = P64::kOut;
+  __shared__ typename P64::JRec jbuf[kWaves][64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int u = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + w));
+  const int count = min(flags[0], cap);
+  const int rw = u / nchunks, c = u - rw * nchunks;
+  const bool active = rw < kRecomputeRows / 64 && rw * 64 < count;
+  const int e = rw * 64 + lane;
+  const int p = active ? flags[1 + min(e, count - 1)] : 0;
+  const typename P64::Params prm{};
+  const typename P64::I q = P64::load_i(si, p, prm);
+  typename P64::A64 a;
+  P64::zero(a);
+  const int jb = c * cw, je = min(nj, jb + cw);
+  for (int j0 = jb; j0 < jb + cw; j0 += 64) {
+    if (active && j0 + lane < je) jbuf[w][lane] = P64::pack(sj, j0 + lane, 0.0);
+    __syncthreads();
+    if (active) {
+      const int nk = min(64, je - j0);
+#pragma unroll 4
+      for (int k = 0; k < nk; ++k) P64::pair(q, jbuf[w][k], a, prm);
+    }
+    __syncthreads();
+  }
+  if (active && e < count) {
+    double v[K];
+    P64::values(a, v);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      rpart[((size_t)c * cap + e) * K + k] = v[k];
+  }
+}
+
+// Outputs of the flagged rows, one block per row (grid-stride): rows below
+// cap add their nchunks partials (each thread a strided subset in chunk
+// order, then a fixed LDS tree); rows beyond cap, if any, are evaluated
+// whole here, the block's threads over the j's.
+template <class P64>
+__global__ __launch_bounds__(256) void sorted_recompute_finish(
+    Soa si, Soa sj, int nj, const int *__restrict__ flags, int nchunks,
+    int cap, const double *__restrict__ rpart,
+    const int *__restrict__ perm_i, Out out) {
+  constexpr int K = P64::kOut;
+  __shared__ double red[K][256];
+  const int t = threadIdx.x;
+  const int count = flags[0];
+  const typename P64::Params prm{};
+  for (int e = blockIdx.x; e < count; e += gridDim.x) {
+    const int p = flags[1 + e];
+    double v[K];
+    if (e < cap) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = 0.0;
+      for (int c = t; c < nchunks; c += 256) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] += rpart[((size_t)c * cap + e) * K + k];
+      }
+    } else {
+      const typename P64::I q = P64::load_i(si, p, prm);
+      typename P64::A64 a;
+      P64::zero(a);
+      for (int j = t; j < nj; j += 256) P64::pair(q, P64::pack(sj, j, 0.0), a, prm);
+      P64::values(a, v);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[k][t] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[k][t] += red[k][t + w];
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[k] = red[k][0];
+      P64::finish(v, out, perm_i[p], prm);
+    }
+    __syncthreads();
+  }
+}
+
+// Byte offsets of the sorted path's scratch, one buffer.
+// The histograms of both sets and the flag count lie in one region that
+// sctx_partial_kernel clears (zero, nzero ints); the flag list follows the
+// count.
+struct SortedLayout {
+  size_t ctxpart, ctx, keys[2][2], vals[2][2], hist[2][kSortPasses], flags,
+      sj, si, origins, boxes, recs, rpart, total;
+  size_t zero, nzero;
+};
+
+template <class Pol>
+SortedLayout sorted_layout(int ni, int nj, bool same, SortGeom gj,
+                           SortGeom gi, int nseg) {
+  SortedLayout L{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  const int ntiles = (nj + kTile - 1) / kTile;
+  L.ctxpart = take(sizeof(double) * kCtxBlocks * kSctxVals);
+  L.ctx = take(sizeof(double) * 2);
+  const int nsets = same ? 1 : 2;
+  for (int s = 0; s < nsets; ++s) {
+    const int n = s == 0 ? nj : ni;
+    for (int b = 0; b < 2; ++b) {
+      L.keys[s][b] = take(sizeof(uint32_t) * n);
+      L.vals[s][b] = take(sizeof(int) * n);
+    }
+  }
+  // histograms and the flag count: contiguous ints (no padding between)
+  L.zero = o;
+  for (int s = 0; s < nsets; ++s) {
+    const SortGeom g = s == 0 ? gj : gi;
+    for (int p = 0; p < kSortPasses; ++p) {
+      L.hist[s][p] = o;
+      o += sizeof(int) * (size_t)kSortBins * g.nb;
+    }
+  }
+  L.flags = o;
+  o += sizeof(int);
+  L.nzero = (o - L.zero) / sizeof(int);
+  o += sizeof(int) * (size_t)ni;  // the flag list
+  o = (o + 255) / 256 * 256;
+  L.sj = take(sizeof(double) * Pol::kFields * (size_t)nj);
+  L.si = same ? L.sj : take(sizeof(double) * Pol::kFields * (size_t)ni);
+  L.origins = take(sizeof(double) * 4 * (size_t)ntiles);
+  L.boxes = take(sizeof(TileBox) * (size_t)ntiles);
+  L.recs = take(sizeof(typename Pol::JRec) * (size_t)nj);
+  L.rpart = take(sizeof(double) * (size_t)kRecomputeRows * nseg * Pol::kOut);
+  // nseg: recompute chunks
+  L.total = o;
+  return L;
+}
+
+}  // namespace tupan
