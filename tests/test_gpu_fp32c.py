@@ -189,6 +189,28 @@ def test_sorted_path_other_sets_duplicates_and_reproducibility():
         assert rel_rows(unsorted, got, grp).max() <= 5e-6, grp
 
 
+def test_sorted_path_nonuniform_softening():
+    """Per-particle softenings (the general far loop, which adds each j's
+    e2; uniform softening takes a loop with the sum precomputed) against
+    the checker, for every FP32C kernel."""
+    n = 8192
+    h = {f: v[:n].copy() for f, v in c2_model(16384).items()}
+    h["eps2"] = h["eps2"] * np.random.RandomState(5).uniform(0.25, 4.0, n)
+    rows = np.arange(0, n, 37)
+    ck = checker()
+    for name, sym in (("phi", "phi_kernel"), ("acc", "acc_kernel"),
+                      ("acc_jerk", "acc_jerk_kernel")):
+        f = spec.KERNELS[name].fields
+        got = kernels.compute(name, {k: h[k] for k in f},
+                              {k: h[k] for k in f}, mode="fp32c")
+        ref = ck.call(sym, {k: h[k][rows] for k in f}, {k: h[k] for k in f})
+        groups = [(0,)] if name == "phi" else [(0, 1, 2), (3, 4, 5)][
+            :len(got) // 3]
+        for grp in groups:
+            e = rel_rows([v[rows] for v in got], ref, grp).max()
+            assert e <= 1e-6, (name, grp, e)
+
+
 @pytest.mark.parametrize("nj", [4097, 4127, 4128, 4129, 12345])
 @pytest.mark.parametrize("ni", [1, 63, 65, 1000])
 def test_sorted_path_sizes(ni, nj):

@@ -105,16 +105,16 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   bool same = n_i == n_j;
   for (int k = 0; k < Pol::kFields; ++k) same = same && is.f[k] == js.f[k];
   const SortGeom gj = sort_geom(n_j), gi = sort_geom(n_i);
-  const int nseg = recompute_chunks(n_j);
+  const int nseg = recompute_groups(n_j);
   const SortedLayout L = sorted_layout<Pol>(n_i, n_j, same, gj, gi, nseg);
   const LaunchShape sh =
       choose_shape(kernel_id, mode, (size_t)n_i, (size_t)n_j, 1, 1, kTile);
-  int S = sh.splits < kWaves ? kWaves : sh.splits;
-  S = (S + kWaves - 1) / kWaves * kWaves;
+  int S = sh.splits < kSWaves ? kSWaves : sh.splits;
+  S = (S + kSWaves - 1) / kSWaves * kSWaves;
   int jchunk = (n_j + S - 1) / S;
   jchunk = (jchunk + kTile - 1) / kTile * kTile;
   const int used = (n_j + jchunk - 1) / jchunk;
-  const int SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
+  const int SB = used > kSWaves ? (used + kSWaves - 1) / kSWaves : 1;
   const int waves = (n_i + 63) / 64;
   if (SB > 65535)
     return set_error(TUPAN_ERR_ARG, "too many j chunks (%d block rows)", SB);
@@ -184,7 +184,7 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   const bool timed = timing_begin(&ev0, &ev1, st);
   if (timed) (void)hipEventRecord(ev0, st);
   hipLaunchKernelGGL((sorted_engine<Pol>), dim3((unsigned)waves, (unsigned)SB),
-                     dim3(kBlock), 0, st, sir, n_i, sjr,
+                     dim3(kSBlock), 0, st, sir, n_i, sjr,
                      (const typename Pol::JRec *)recs, n_j, boxes, jchunk, part,
                      (const double *)origins);
   if (timed) (void)hipEventRecord(ev1, st);
@@ -193,11 +193,11 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
                      out, flag_tau() * flag_tau(), flags);
   double *rpart = (double *)at(L.rpart);
   {
-    const int cw = (n_j + nseg - 1) / nseg;
-    const int items = (kRecomputeRows / 64) * nseg;
+    // nseg groups of kWaves chunks of cw j's
+    const int cw = (n_j + kWaves * nseg - 1) / (kWaves * nseg);
     hipLaunchKernelGGL((sorted_recompute_kernel<P64>),
-                       dim3((items + kWaves - 1) / kWaves), dim3(kBlock), 0, st,
-                       sir, sjr, n_j, (const int *)flags, cw, nseg,
+                       dim3((kRecomputeRows / 64) * nseg), dim3(kBlock), 0,
+                       st, sir, sjr, n_j, (const int *)flags, cw, nseg,
                        kRecomputeRows, rpart);
     hipLaunchKernelGGL((sorted_recompute_finish<P64>), dim3(kRecomputeBlocks),
                        dim3(256), 0, st, sir, sjr, n_j, (const int *)flags,

@@ -42,8 +42,9 @@
 //                         kFlagTau * |result| (tau^2 on squares)
 //   sorted_recompute_kernel + sorted_recompute_finish
 //                         the flagged rows again in FP64 (the FP64 policy):
-//                         64 rows per wave over j chunks, then the chunk
-//                         partials of each row added in a fixed order:
+//                         64 rows per wave over j chunks (4 per block,
+//                         combined in LDS), then the partials of each row
+//                         added in a fixed order:
 //                         bit-reproducible whatever order the rows were
 //                         flagged in
 // No memset anywhere (the clears are kernels), so a call can be captured.
@@ -55,7 +56,7 @@ namespace tupan {
 constexpr int kSortDigit = 10;                 // bits per radix pass
 constexpr int kSortBins = 1 << kSortDigit;
 constexpr int kSortPasses = 3;                 // 30-bit keys
-constexpr int kSortChunk = 2048;               // keys per block (minimum)
+constexpr int kSortChunk = 1024;               // keys per block (minimum)
 constexpr int kSortMaxBlocks = 256;
 constexpr int kSctxVals = 14;                  // per-block context values
 // Rows whose estimated FP32 error exceeds this fraction of |result| are
@@ -68,14 +69,15 @@ constexpr int kSctxVals = 14;                  // per-block context values
 constexpr double kFlagTau = 0.7;
 constexpr int kRecomputeRows = 1024;  // flagged rows recomputed in chunks
 constexpr int kRecomputeMinJ = 64;    // j's per recompute chunk (minimum)
-constexpr int kRecomputeChunks = 1024;
+constexpr int kRecomputeGroups = 256; // chunk groups of 4 (one per block)
 constexpr int kRecomputeBlocks = 1024;  // sorted_recompute_finish
 
-// Chunks of the FP64 recomputation: at most kRecomputeChunks of at least
-// kRecomputeMinJ j's, so that the few row-waves still fill the chip.
-inline int recompute_chunks(int nj) {
-  int n = (nj + kRecomputeMinJ - 1) / kRecomputeMinJ;
-  return n < 1 ? 1 : (n > kRecomputeChunks ? kRecomputeChunks : n);
+// Chunk groups of the FP64 recomputation: at most kRecomputeGroups of 4
+// chunks of at least kRecomputeMinJ j's each, so that the few row-waves
+// still fill the chip.
+inline int recompute_groups(int nj) {
+  int n = (nj + 4 * kRecomputeMinJ - 1) / (4 * kRecomputeMinJ);
+  return n < 1 ? 1 : (n > kRecomputeGroups ? kRecomputeGroups : n);
 }
 
 // Geometry of one radix sort: nb blocks of ch keys (ch a multiple of 256).
@@ -132,7 +134,7 @@ __device__ __forceinline__ double tile_max(double v) {
 
 // Per block b: [0] sum m_j, [1] max e2_i, [2..7] j box (lo xyz, hi xyz),
 // [8..13] i box.  Grid kCtxBlocks x 256.  Also clears `nzero` ints at
-// `zero` (histograms of passes 1..2, the flag count).
+// `zero` (the radix histograms, the flag count).
 static __global__ __launch_bounds__(256) void sctx_partial_kernel(
     Soa js, int nj, Soa is, int ni, double *__restrict__ part,
     int *__restrict__ zero, int nzero) {
@@ -439,8 +441,17 @@ __global__ __launch_bounds__(256) void sorted_gather_kernel(
 // pair_engine's FP32C path (kTwoLevel policies, one i per lane) over
 // sorted particles.  Always writes its block-row partials (outputs, then
 // the error statistics) for sorted_finish_kernel.
+// Waves per block of the sorted engine: a block's waves take the same 64
+// i's and consecutive j chunks and combine their sums through LDS, so the
+// block rows (and the partial sums sorted_finish_kernel reads) are S / 8.
+constexpr int kSWaves = 4;
+constexpr int kSBlock = 64 * kSWaves;
+
+// 72 VGPRs (7 waves per SIMD; a few per-tile values spill to scratch
+// outside the pair loops): measured faster than the 82 registers and 5
+// waves the compiler picks by itself.
 template <class Pol>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void sorted_engine(
+__global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void sorted_engine(
     Soa si, int ni, Soa sj, const typename Pol::JRec *__restrict__ jr, int nj,
     const TileBox *__restrict__ boxes, int jchunk,
     double *__restrict__ part, const double *__restrict__ aux) {
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) 
   const int row = blockIdx.y;
   const int i0 = blockIdx.x * 64;
   if (i0 >= ni) return;  // block-uniform
-  const int s = row * kWaves + wave;
+  const int s = row * kSWaves + wave;
   const int i = min(i0 + lane, ni - 1);
   const typename Pol::Params prm{};
   typename Pol::I q = Pol::load_i(si, i, prm);
@@ -543,14 +554,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) 
   Pol::values(acc, v);
 #pragma unroll
   for (int g = 0; g < Pol::kGroups; ++g) v[Pol::kOut + g] = (double)sig[g];
-  __shared__ double red[kWaves - 1][kPart][64];
+  __shared__ double red[kSWaves - 1][kPart][64];
   if (wave > 0) {
 #pragma unroll
     for (int k = 0; k < kPart; ++k) red[wave - 1][k][lane] = v[k];
   }
   __syncthreads();
   if (wave != 0) return;
-  for (int w = 0; w < kWaves - 1; ++w) {
+  for (int w = 0; w < kSWaves - 1; ++w) {
 #pragma unroll
     for (int k = 0; k < kPart; ++k) v[k] += red[w][k][lane];
   }
@@ -599,29 +610,30 @@ __global__ __launch_bounds__(256) void sorted_finish_kernel(
 // walks one chunk of cw j's, so the j data is read once per 64 rows.  The
 // chunk passes through LDS 64 records at a time (one coalesced load per
 // lane, then wave-uniform LDS reads), so a wave waits for memory once per
-// 64 pairs rather than once per pair.  Work item u = (row-wave u / nchunks,
-// chunk u % nchunks) for the first min(count, cap) flagged rows; waves
-// without rows still take part in the block's barriers.  Partials go to
-// rpart[(chunk * cap + row) * kOut + k].
+// 64 pairs rather than once per pair.  A block's 4 waves take the same 64
+// rows and 4 consecutive chunks and add their sums in LDS in wave order;
+// block b = (row-wave b / ngroups, chunk group b % ngroups) writes
+// rpart[(row * ngroups + group) * kOut + k] for the first min(count, cap)
+// flagged rows.  Waves without rows still take part in the barriers.
 template <class P64>
 __global__ __launch_bounds__(kBlock) void sorted_recompute_kernel(
     Soa si, Soa sj, int nj, const int *__restrict__ flags, int cw,
-    int nchunks, int cap, double *__restrict__ rpart) {
+    int ngroups, int cap, double *__restrict__ rpart) {
   constexpr int K = P64::kOut;
   __shared__ typename P64::JRec jbuf[kWaves][64];
+  __shared__ double red[kWaves - 1][K][64];
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int u = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWaves + w));
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int count = min(flags[0], cap);
-  const int rw = u / nchunks, c = u - rw * nchunks;
-  const bool active = rw < kRecomputeRows / 64 && rw * 64 < count;
+  const int rw = blockIdx.x / ngroups, grp = blockIdx.x - rw * ngroups;
+  const bool active = rw * 64 < count;  // block-uniform
   const int e = rw * 64 + lane;
   const int p = active ? flags[1 + min(e, count - 1)] : 0;
   const typename P64::Params prm{};
   const typename P64::I q = P64::load_i(si, p, prm);
   typename P64::A64 a;
   P64::zero(a);
-  const int jb = c * cw, je = min(nj, jb + cw);
+  const int jb = (grp * kWaves + w) * cw, je = min(nj, jb + cw);
   for (int j0 = jb; j0 < jb + cw; j0 += 64) {
     if (active && j0 + lane < je) jbuf[w][lane] = P64::pack(sj, j0 + lane, 0.0);
     __syncthreads();
@@ -632,22 +644,31 @@ __global__ __launch_bounds__(kBlock) void sorted_recompute_kernel(
     }
     __syncthreads();
   }
-  if (active && e < count) {
-    double v[K];
-    P64::values(a, v);
+  if (!active) return;  // block-uniform, after the last barrier
+  double v[K];
+  P64::values(a, v);
+  if (w > 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-      rpart[((size_t)c * cap + e) * K + k] = v[k];
+    for (int k = 0; k < K; ++k) red[w - 1][k][lane] = v[k];
   }
+  __syncthreads();
+  if (w != 0 || e >= count) return;
+  for (int o = 0; o < kWaves - 1; ++o) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += red[o][k][lane];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    rpart[((size_t)e * ngroups + grp) * K + k] = v[k];
 }
 
 // Outputs of the flagged rows, one block per row (grid-stride): rows below
-// cap add their nchunks partials (each thread a strided subset in chunk
+// cap add their ngroups partials (thread t the groups t, t + 256, ... in
 // order, then a fixed LDS tree); rows beyond cap, if any, are evaluated
 // whole here, the block's threads over the j's.
 template <class P64>
 __global__ __launch_bounds__(256) void sorted_recompute_finish(
-    Soa si, Soa sj, int nj, const int *__restrict__ flags, int nchunks,
+    Soa si, Soa sj, int nj, const int *__restrict__ flags, int ngroups,
     int cap, const double *__restrict__ rpart,
     const int *__restrict__ perm_i, Out out) {
   constexpr int K = P64::kOut;
@@ -661,9 +682,10 @@ __global__ __launch_bounds__(256) void sorted_recompute_finish(
     if (e < cap) {
 #pragma unroll
       for (int k = 0; k < K; ++k) v[k] = 0.0;
-      for (int c = t; c < nchunks; c += 256) {
+      for (int c = t; c < ngroups; c += 256) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += rpart[((size_t)c * cap + e) * K + k];
+        for (int k = 0; k < K; ++k)
+          v[k] += rpart[((size_t)e * ngroups + c) * K + k];
       }
     } else {
       const typename P64::I q = P64::load_i(si, p, prm);
@@ -742,7 +764,7 @@ SortedLayout sorted_layout(int ni, int nj, bool same, SortGeom gj,
   L.boxes = take(sizeof(TileBox) * (size_t)ntiles);
   L.recs = take(sizeof(typename Pol::JRec) * (size_t)nj);
   L.rpart = take(sizeof(double) * (size_t)kRecomputeRows * nseg * Pol::kOut);
-  // nseg: recompute chunks
+  // nseg: chunk groups of the FP64 recomputation
   L.total = o;
   return L;
 }
