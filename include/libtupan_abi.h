@@ -1,7 +1,7 @@
 /*
  * libtupan_abi.h -- the drop-in libraries libtupan_f64.so and
  * libtupan_f32.so.  They export exactly the kernel symbols of the
- * reference's tupan/lib/src/libtupan.h:2-229 with the same argument lists,
+ * reference's tupan/lib/src/libtupan.h:2-246 with the same argument lists,
  * for REAL = double, INT = long, UINT = unsigned long (libtupan_f64.so,
  * compile users with -DTUPAN_ABI_F64) or REAL = float, INT = int,
  * UINT = unsigned int (libtupan_f32.so), as the reference builds its two
@@ -11,9 +11,11 @@
  * in that library's default mode (tupan_set_default_mode; FP64 unless set).
  * The reference functions report nothing; on failure these write NaN to
  * every output and tupan_abi_last_status() returns the TUPAN_ERR_* code of
- * the last call (0 on success).  The out-of-scope nreg_Xkernel,
- * nreg_Vkernel and kepler_solver_kernel (libtupan.h:152-201, 231-246) are
- * not provided.
+ * the last call: 0 on success, TUPAN_ERR_NOT_CONVERGED when a Sakura or
+ * Kepler solve hit its cap (results written), TUPAN_ERR_NONFINITE when the
+ * results hold NaN or inf (results written).  All ten libtupan.h kernels are
+ * provided, nreg_Xkernel, nreg_Vkernel and kepler_solver_kernel
+ * (libtupan.h:152-201, 231-246) included.
  */
 #ifndef LIBTUPAN_ABI_H
 #define LIBTUPAN_ABI_H

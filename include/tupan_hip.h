@@ -53,6 +53,11 @@ extern "C" {
 #define TUPAN_ERR_HIP 1           /* HIP runtime failure (no device, OOM) */
 #define TUPAN_ERR_ARG 2           /* invalid argument (mode, NULL, size) */
 #define TUPAN_ERR_NOT_CONVERGED 3 /* Sakura: a Kepler solve hit its cap */
+#define TUPAN_ERR_NONFINITE 4     /* drop-in status: an output is NaN/inf */
+
+/* Bits of a device status word (d_status arguments) */
+#define TUPAN_STATUS_NOT_CONVERGED 1  /* a Kepler solve hit its cap */
+#define TUPAN_STATUS_NONFINITE 2      /* tupan_check_finite_dev found NaN/inf */
 
 #define TUPAN_MODE_FP64 0
 #define TUPAN_MODE_FP32C 1
@@ -456,6 +461,14 @@ int tupan_report_terms_dev(size_t n, const double *const *f, int pn,
                            double *const *out, void *stream);
 int tupan_diagnostics_dev(size_t n, const double *const *f, int pn,
                           double *out, void *stream);
+
+/* ---- non-finite outputs -------------------------------------------------
+ * No counterpart in the reference (its kernels report nothing).  OR-s
+ * TUPAN_STATUS_NONFINITE into the device int *d_status when any of
+ * arrays[0..narrays)[0..n) is NaN or infinite; leaves it alone otherwise.
+ * Asynchronous on the stream, allocates nothing, records no memset. */
+int tupan_check_finite_dev(size_t n, int narrays, const double *const *arrays,
+                           int *d_status, void *stream);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,7 @@ def lib():
         L.tupan_set_sorted.argtypes = [ctypes.c_int]
         L.tupan_last_flagged.argtypes = [ctypes.POINTER(ctypes.c_int)]
         L.tupan_set_flag_tau.argtypes = [ctypes.c_double]
+        L.tupan_check_finite_dev.argtypes = [_SZ, ctypes.c_int, _PP, _P, _P]
         L.tupan_set_sakura_budget.argtypes = [ctypes.c_int]
         L.tupan_take_kernel_times.argtypes = [ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_int)]
@@ -278,6 +279,17 @@ def call_host(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=()):
     rc = fn(*args)
     del keep
     check(rc)
+
+
+STATUS_NOT_CONVERGED = 1
+STATUS_NONFINITE = 2
+
+
+def check_finite_dev(n, ptrs, status_ptr, stream=0):
+    """OR STATUS_NONFINITE into the device int at status_ptr when any of
+    the n-element device arrays holds NaN or inf (asynchronous)."""
+    check(lib().tupan_check_finite_dev(_SZ(n), len(ptrs), ptr_array(ptrs),
+                                       _P(status_ptr), _P(stream or 0)))
 
 
 def call_dev(name, mode, ni, iptrs, nj, jptrs, optrs, scalars=(), stream=0,

@@ -17,7 +17,8 @@
  *
  * The reference functions return void and report nothing.  On a device or
  * argument failure these shims write NaN to every output, print the error
- * once to stderr and record it for tupan_abi_last_status().
+ * once to stderr and record it for tupan_abi_last_status(); a call whose
+ * results hold NaN or inf records TUPAN_ERR_NONFINITE there (results kept).
  */
 #include <math.h>
 #include <stdio.h>
@@ -35,7 +36,17 @@ EXPORT int tupan_abi_last_status(void) { return g_status; }
 static void report(int rc, const char *who, size_t ni, REAL **outs, int nout)
 {
     g_status = rc;
-    if (rc == TUPAN_OK) return;
+    if (rc == TUPAN_OK) {
+        /* results written: note NaN/inf among them (the reference would
+         * return them silently) */
+        for (int k = 0; k < nout && g_status == TUPAN_OK; ++k)
+            for (size_t i = 0; i < ni; ++i)
+                if (!isfinite(outs[k][i])) {
+                    g_status = TUPAN_ERR_NONFINITE;
+                    break;
+                }
+        return;
+    }
     static int printed = 0;
     if (!printed) {
         fprintf(stderr, "%s: %s\n", who, tupan_last_error());

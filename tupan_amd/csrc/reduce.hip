@@ -251,12 +251,48 @@ __global__ __launch_bounds__(256) void energy_terms_kernel(
   }
 }
 
+// Lane-parallel scan of up to 8 arrays for NaN/inf; a block that finds one
+// ORs the flag into the status word from one lane (vector atomic).
+struct FinitePtrs {
+  const double *a[8];
+};
+__global__ __launch_bounds__(256) void check_finite_kernel(
+    int n, int na, FinitePtrs p, int *status) {
+  bool bad = false;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    for (int k = 0; k < na; ++k) bad = bad || !isfinite(p.a[k][i]);
+  if (__syncthreads_or(bad) && threadIdx.x == 0)
+    atomicOr(status, TUPAN_STATUS_NONFINITE);
+}
+
 }  // namespace
 }  // namespace tupan
 
 using namespace tupan;
 
 extern "C" {
+
+int tupan_check_finite_dev(size_t n, int narrays, const double *const *arrays,
+                           int *d_status, void *stream) {
+  if (narrays < 0 || narrays > 8)
+    return set_error(TUPAN_ERR_ARG, "narrays must be in 0..8 (got %d)",
+                     narrays);
+  if (!d_status) return set_error(TUPAN_ERR_ARG, "NULL status");
+  if (n > (size_t)INT_MAX) return set_error(TUPAN_ERR_ARG, "n too large");
+  if (n == 0 || narrays == 0) return TUPAN_OK;
+  if (!arrays) return set_error(TUPAN_ERR_ARG, "NULL array list");
+  FinitePtrs p{};
+  for (int k = 0; k < narrays; ++k) {
+    if (!arrays[k]) return set_error(TUPAN_ERR_ARG, "NULL array %d", k);
+    p.a[k] = arrays[k];
+  }
+  const int blocks = (int)std::min<size_t>(256, (n + 255) / 256);
+  hipLaunchKernelGGL(check_finite_kernel, dim3(blocks), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), (int)n, narrays, p,
+                     d_status);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? TUPAN_OK : hip_fail(e, "check_finite_kernel");
+}
 
 int tupan_sum_dev(size_t n, const double *x, double *out, void *stream) {
   if (n && !x) return set_error(TUPAN_ERR_ARG, "NULL field");

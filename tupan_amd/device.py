@@ -31,8 +31,10 @@ def compute(name, ips, jps, *args, mode="fp64", out=None, stream=None,
             status=None):
     """Evaluate kernel ``name``; returns the tuple of output tensors.
 
-    ``status`` (sakura only): optional int32 device tensor of length 1 that
-    is OR-ed with 1 when a Kepler solve hit its cap.
+    ``status``: optional int32 device tensor of length 1, OR-ed on the
+    device (no synchronisation) with 1 when a Sakura Kepler solve hit its
+    cap and with 2 when an output holds NaN or inf
+    (tupan_check_finite_dev).
     """
     k = spec.KERNELS[name]
     it = [_field(ips, f) for f in k.fields]
@@ -58,6 +60,8 @@ def compute(name, ips, jps, *args, mode="fp64", out=None, stream=None,
     _lib.call_dev(name, mode, ni, [t.data_ptr() for t in it], nj,
                   [t.data_ptr() for t in jt], [o.data_ptr() for o in out],
                   args, stream=st, status_ptr=sptr)
+    if sptr is not None:
+        _lib.check_finite_dev(ni, [o.data_ptr() for o in out], sptr, st)
     return out
 
 

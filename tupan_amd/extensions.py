@@ -128,15 +128,27 @@ class HIPKernel(object):
         if self.key == "pnacc":
             scalars = (scalars[0], tuple(scalars[1:8]))
         mode = self.mode or default_mode()
-        wide = self.real != np.float64
+        for x in tuple(iarr) + tuple(outs):
+            if len(x) < ni:
+                raise ValueError("%s: array shorter than ni = %d"
+                                 % (self.name, ni))
+        for x in jarr:
+            if len(x) < nj:
+                raise ValueError("%s: array shorter than nj = %d"
+                                 % (self.name, nj))
         iw = [_lib.as_f64(x[:ni]) for x in iarr]
         jw = [_lib.as_f64(x[:nj]) for x in jarr]
-        ow = [np.empty(ni, np.float64) if wide else o for o in outs]
+        # outputs the device can write directly: contiguous float64; any
+        # other (float32, strided views) goes through a contiguous buffer
+        direct = [o.dtype == np.float64 and o.flags.c_contiguous
+                  for o in outs]
+        ow = [o if d else np.empty(ni, np.float64)
+              for o, d in zip(outs, direct)]
         _lib.call_host(self.key, mode, ni, [x.ctypes.data for x in iw], nj,
                        [x.ctypes.data for x in jw],
                        [o.ctypes.data for o in ow], scalars)
-        if wide:
-            for o, w in zip(outs, ow):
+        for o, w, d in zip(outs, ow, direct):
+            if not d:
                 o[:ni] = w
 
 
