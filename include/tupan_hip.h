@@ -155,6 +155,23 @@ int tupan_acc_jerk_host(int mode, size_t ni, const double *const *ip,
                         size_t nj, const double *const *jp,
                         double *const *op);
 
+/* ---- AccJerk without allocation -------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  tupan_acc_jerk_dev in TUPAN_MODE_FP64 (same fields, outputs
+ * and launch shape; the same bits whenever the j records and block-row
+ * partials fit) whose scratch is a file-scope 32 MiB device array, one per
+ * device, held until the process ends: the call allocates and frees
+ * nothing.  The j split shrinks until the records (64 B per j, none for
+ * nj <= 4096) and partials (48 B per i per block row) fit; a call whose
+ * records alone exceed 32 MiB (nj > 524288) is refused with TUPAN_ERR_ARG.
+ * Every call on a device shares that array: calls that overlap on one
+ * device (different streams or host threads) race and must be serialised
+ * by the caller.  Nothing in the array needs clearing between calls. */
+int tupan_acc_jerk_static_dev(size_t ni, const double *const *ip,
+                              size_t nj, const double *const *jp,
+                              double *const *op, void *stream);
+
 /* ---- SnapCrackle: replaces snap_crackle_kernel, libtupan.h:60-96 -----
  * ip, jp: F14.  op: sx, sy, sz, cx, cy, cz. */
 int tupan_snap_crackle_dev(int mode, size_t ni, const double *const *ip,

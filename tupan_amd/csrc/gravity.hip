@@ -26,6 +26,8 @@
 //               tupan/lib/src/tstep_kernel.c:46-47)
 // with the softening of tupan/lib/src/smoothing.h:10-184: s^2 = r^2 + e2_i +
 // e2_j, and a pair contributes nothing unless the *unsoftened* r^2 > 0.
+#include <mutex>
+
 #include "accjerk.hpp"
 #include "f32c.hpp"
 #include "launch.hpp"
@@ -353,6 +355,38 @@ struct TstepF64 {
   }
 };
 
+// ------------------------------------------------ static scratch
+// Device memory of tupan_acc_jerk_static_dev: the j records and block-row
+// partials of its FP64 AccJerk live in this file-scope array instead of
+// the scratch cache, so that the call allocates nothing.  Each device holds
+// its own copy for the life of the process.  In host code the array's name
+// is not a device address: its address on each device comes from
+// hipGetSymbolAddress (static_scratch), and only that is passed to kernels.
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_static_mu;
+static void *g_static_ptr[kMaxDevices];
+
+// The array's address on the current device, looked up once per device.
+static int static_scratch(void **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(TUPAN_ERR_ARG, "device %d beyond the static table", dev);
+  std::lock_guard<std::mutex> lk(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void *q = nullptr;
+    e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_static_ptr[dev] = q;
+  }
+  *ptr = g_static_ptr[dev];
+  return TUPAN_OK;
+}
+
 }  // namespace tupan
 
 using namespace tupan;
@@ -449,6 +483,25 @@ int tupan_acc_jerk_host(int mode, size_t ni, const double *const *ip,
                         double *const *op) {
   return dispatch_host<AccJerkF64, AccJerkF32c>(TUPAN_K_ACC_JERK, mode, 8, 6,
                                                 ni, ip, nj, jp, op);
+}
+
+int tupan_acc_jerk_static_dev(size_t ni, const double *const *ip,
+                              size_t nj, const double *const *jp,
+                              double *const *op, void *stream) {
+  // the j records alone must fit (nj above the direct-record bound)
+  if (nj > (size_t)kDirectMaxJ &&
+      sizeof(AccJerkF64::JRec) * nj > kStaticScratchBytes)
+    return set_error(TUPAN_ERR_ARG,
+                     "nj = %zu needs %zu bytes of j records; the static "
+                     "scratch holds %zu", nj, sizeof(AccJerkF64::JRec) * nj,
+                     kStaticScratchBytes);
+  if (ni == 0) return TUPAN_OK;
+  void *ptr = nullptr;
+  const int rc = static_scratch(&ptr);
+  if (rc) return rc;
+  const FixedScratch fixed{static_cast<char *>(ptr), kStaticScratchBytes};
+  return run_engine<AccJerkF64>(TUPAN_K_ACC_JERK, TUPAN_MODE_FP64, ni, ip, nj,
+                                jp, op, {}, as_stream(stream), &fixed);
 }
 
 int tupan_snap_crackle_dev(int mode, size_t ni, const double *const *ip,

@@ -272,10 +272,19 @@ __global__ __launch_bounds__(256) void small_pack_kernel(
     jr[j] = Pol::pack(js, j, PackCtx{factor, e2max, org});
 }
 
+// A caller-owned device buffer that replaces the scratch cache for one
+// call (the no-allocation entry points): plain FP64 policies only; the j
+// split is reduced until the records and block-row partials fit.
+struct FixedScratch {
+  char *base;
+  size_t bytes;
+};
+
 template <class Pol>
 int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
                size_t nj, const double *const *jp, double *const *op,
-               const typename Pol::Params &prm, hipStream_t st) {
+               const typename Pol::Params &prm, hipStream_t st,
+               const FixedScratch *fixed = nullptr) {
   if (ni == 0) return TUPAN_OK;
   if (!ip || !jp || !op) return set_error(TUPAN_ERR_ARG, "NULL field array");
   if (ni > (size_t)INT_MAX / 8 || nj > (size_t)INT_MAX / 8)
@@ -313,8 +322,8 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   int jchunk = (n_j + S - 1) / S;
   jchunk = (jchunk + kChunk - 1) / kChunk * kChunk;
   if (jchunk < kChunk) jchunk = kChunk;
-  const int used = (n_j + jchunk - 1) / jchunk;  // chunks holding records
-  const int SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
+  int used = (n_j + jchunk - 1) / jchunk;  // chunks holding records
+  int SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
   const int waves = (n_i + 64 * P - 1) / (64 * P);  // i-waves
   if (SB > 65535)
     return set_error(TUPAN_ERR_ARG, "too many j chunks (%d block rows)", SB);
@@ -331,14 +340,38 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
                      : 0;
   // FP64 policies at small nj build their records in the pair kernel
   const bool direct = !Pol::kTwoLevel && n_j <= kDirectMaxJ;
-  if (n_j > 0 && !direct) {
+  if (fixed) {
+    if (Pol::kTwoLevel || SparsePairs<Pol>::value)
+      return set_error(TUPAN_ERR_ARG, "fixed scratch: FP64 policies only");
+    const size_t rec = direct ? 0 : rec_bytes;
+    auto part_bytes = [&](int sb) -> size_t {
+      return sb > 1 ? ((sizeof(double) * (size_t)sb * Pol::kOut * n_i + 255) /
+                       256) * 256
+                    : 0;
+    };
+    // one block row fewer at a time until records and partials fit
+    while (SB > 1 && rec + part_bytes(SB) > fixed->bytes) {
+      S = (SB - 1) * kWaves;
+      jchunk = (n_j + S - 1) / S;
+      jchunk = (jchunk + kChunk - 1) / kChunk * kChunk;
+      used = (n_j + jchunk - 1) / jchunk;
+      SB = used > kWaves ? (used + kWaves - 1) / kWaves : 1;
+    }
+    if (rec + part_bytes(SB) > fixed->bytes)
+      return set_error(TUPAN_ERR_ARG,
+                       "the call needs %zu bytes of scratch; the fixed "
+                       "buffer holds %zu", rec + part_bytes(SB),
+                       fixed->bytes);
+    if (rec) jr = reinterpret_cast<typename Pol::JRec *>(fixed->base);
+    if (SB > 1) part = reinterpret_cast<double *>(fixed->base + rec);
+  } else if (n_j > 0 && !direct) {
     rc = scratch_alloc(kSlotRecords, (void **)&jr, rec_bytes + ctx_bytes, st);
     if (rc) return rc;
     if (Pol::kTwoLevel)
       ctx = reinterpret_cast<double *>(reinterpret_cast<char *>(jr) +
                                        rec_bytes);
   }
-  if (SB > 1) {
+  if (SB > 1 && !fixed) {
     rc = scratch_alloc(kSlotPartials, (void **)&part,
                        sizeof(double) * (size_t)SB * Pol::kOut * (size_t)n_i,
                        st);
@@ -398,7 +431,7 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   dim3 grid((unsigned)waves, (unsigned)SB);
   const double *aux = ctx ? ctx + 2 : nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  const bool timed = timing_begin(&ev0, &ev1, st);
+  const bool timed = !fixed && timing_begin(&ev0, &ev1, st);
   if (timed) (void)hipEventRecord(ev0, st);
   switch (P) {
     case 4:
@@ -419,8 +452,10 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
                        dim3(256), 0, st, part, SB, n_i, out, prm);
   hipError_t e = hipGetLastError();
   if (mask) scratch_free(kSlotMask, mask, st);
-  scratch_free(kSlotPartials, part, st);
-  scratch_free(kSlotRecords, jr, st);
+  if (!fixed) {
+    scratch_free(kSlotPartials, part, st);
+    scratch_free(kSlotRecords, jr, st);
+  }
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return TUPAN_OK;
 }
