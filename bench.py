@@ -67,6 +67,8 @@ def parse():
                    help="pn and nreg workloads: replay each step as a HIP "
                         "graph")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--sizes", default="16384,65536,262144",
+                   help="sizes workload: the N values (comma-separated)")
     p.add_argument("--workload", default="accjerk",
                    choices=["accjerk", "c1", "sizes", "ensemble", "pn", "diag",
                             "nreg", "adjoint"],
@@ -346,11 +348,13 @@ def bench_sizes(args, torch):
     seed 1, Salpeter masses [0.1, 100], eps = 4/N.  Phi, Acc, AccJerk and
     SnapCrackle (its a, j inputs from the FP64 AccJerk) in FP64 and FP32C,
     Tstep (eta = 0.02) in FP64 (its only mode): interactions/s from the live
-    engine timing, and per-particle relative errors |x - x64| / |x64| (max,
-    RMS) of every FP32C output against the FP64 kernel."""
-    from tupan_amd import device, spec
+    engine timing and from the whole call (events around it), the FP32C
+    rows recomputed in FP64, and per-particle relative errors
+    |x - x64| / |x64| (max, RMS) of every FP32C output against the FP64
+    kernel."""
+    from tupan_amd import _lib, device, spec
     rows = []
-    for n in (16384, 65536, 262144):
+    for n in [int(x) for x in args.sizes.split(",")]:
         print("[bench] sizes: N=%d" % n, file=sys.stderr, flush=True)
         ps = make_workload(n)
         d = device.to_device(ps, spec.F8)
@@ -368,13 +372,20 @@ def bench_sizes(args, torch):
                                               out=out)
                 reps = max(2, min(20, int(2e11 / (n * n))))
                 _, k_ms, _ = time_steps(step, reps, 1, torch)
-                res[mode] = (out, k_ms)
+                call_ms = float(np.median(time_steps.step_ms))
+                res[mode] = (out, k_ms, call_ms)
             pairs = float(n) * n
             row = {"n": n, "kernel": name,
                    "flops_per_interaction": spec.KERNELS[name].flops}
             for mode in modes:
                 row[mode + "_Gint_s"] = pairs / (res[mode][1] * 1e-3) / 1e9
                 row[mode + "_kernel_ms"] = res[mode][1]
+                # the whole call: for FP32C the sort, packing, engine,
+                # finish and the FP64 recomputation of the flagged rows
+                row[mode + "_call_ms"] = res[mode][2]
+                row[mode + "_call_Gint_s"] = pairs / (res[mode][2] * 1e-3) / 1e9
+            if "fp32c" in res:
+                row["fp32c_rows_recomputed_fp64"] = _lib.last_flagged()
             if "fp32c" not in res:
                 rows.append(row)
                 print(json.dumps(row), file=sys.stderr, flush=True)
@@ -394,7 +405,10 @@ def bench_sizes(args, torch):
                 row[tag + "_rms"] = float(e.pow(2).mean().sqrt())
             rows.append(row)
             print(json.dumps(row), file=sys.stderr, flush=True)
-    big = [r for r in rows if r["n"] == 65536 and r["kernel"] == "acc_jerk"][0]
+    big = [r for r in rows if r["kernel"] == "acc_jerk"][-1]
+    for r in rows:
+        if r["n"] == 65536 and r["kernel"] == "acc_jerk":
+            big = r
     result = {
         "metric": "single-evaluation kernels at N=16K/64K/256K, FP32C and "
                   "FP64, Ginteractions/s and errors vs FP64",

@@ -70,7 +70,18 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
     double x, y, z, vx, vy, vz;
     float e2, dvx, dvy, dvz;
     int i;
+    float px, py, pz;  // position offsets to the tile's origin (far tiles)
   };
+  // far-tile record: one 32-byte scalar load per pair
+  static constexpr bool kFarRec = true;
+  struct alignas(32) FarRec {
+    float x, y, z, dvx, dvy, dvz, m, e2;
+  };
+  // the far record of r, whose tile has the position origin po
+  __device__ static FarRec far_rec(const JRec &r, const double *po) {
+    return {(float)(r.x - po[0]), (float)(r.y - po[1]), (float)(r.z - po[2]),
+            r.dvx, r.dvy, r.dvz, r.m, r.e2};
+  }
   // record of j with the velocity origin o (3 doubles) of its tile
   __device__ static JRec pack_o(const Soa &s, int j, const PackCtx &c,
                                 const double *o) {
@@ -101,7 +112,8 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
     q.dvy = (float)((q.vy - o[1]) * vs);
     q.dvz = (float)((q.vz - o[2]) * vs);
   }
-  __device__ static void body32(const I &q, const JRec &r, float rx, float ry,
+  template <class R>  // JRec or FarRec
+  __device__ static void body32(const I &q, const R &r, float rx, float ry,
                                 float rz, float s2, bool far, A32 &a) {
     float vx = q.dvx - r.dvx, vy = q.dvy - r.dvy, vz = q.dvz - r.dvz;
     float rv = rx * vx + ry * vy + rz * vz;

@@ -61,6 +61,8 @@ struct F32cBase : SumOut<N> {
   static constexpr bool kTwoLevel = true;
   static constexpr int kVelOrigins = 0;
   using Exact = P64;  // the FP64 policy (near pairs, recomputed rows)
+  // minimum waves per SIMD the sorted engine is compiled for
+  static constexpr int kWavesPerEu = 7;
   using Params = NoParams;
   using A64 = F64Acc<N>;
   using A32 = F32Acc<N>;
@@ -116,6 +118,31 @@ struct F32cBase : SumOut<N> {
   __device__ static void pair32_far(const I &q, const JRec &r, A32 &a) {
     const float rx = dsub32(q.x, r.x), ry = dsub32(q.y, r.y),
                 rz = dsub32(q.z, r.z);
+    const float s2 = __builtin_fmaf(
+        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+    Derived::body32(q, r, rx, ry, rz, s2, true, a);
+  }
+  // Far tiles from FarRec records (policies with kFarRec, sorted engine):
+  // the positions are FP32 offsets to the j tile's FP64 position origin
+  // (the centre of its bounding box), the j offset rounded once when
+  // packed and the i offset once per tile (tile_begin_far), so a pair forms
+  // its separation with three FP32 subtractions in place of dsub32's three
+  // FP64 subtractions and conversions.  On far tiles both offsets are about
+  // as large as the separation, so the two roundings add an error of the
+  // order of dsub32's one.
+  static constexpr bool kFarRec = false;
+  struct FarRec {
+    float x, y, z, e2;
+  };
+  template <class I>
+  __device__ static void tile_begin_far(I &q, const double *po) {
+    q.px = (float)(q.x - po[0]);
+    q.py = (float)(q.y - po[1]);
+    q.pz = (float)(q.z - po[2]);
+  }
+  template <class I, class R>
+  __device__ static void pair32_farrec(const I &q, const R &r, A32 &a) {
+    const float rx = q.px - r.x, ry = q.py - r.y, rz = q.pz - r.z;
     const float s2 = __builtin_fmaf(
         rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
     Derived::body32(q, r, rx, ry, rz, s2, true, a);

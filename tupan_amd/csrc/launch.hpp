@@ -173,9 +173,12 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   double *origins = (double *)at(L.origins);
   const TileBox *boxes = (const TileBox *)at(L.boxes);
   auto *recs = (typename Pol::JRec *)at(L.recs);
+  double *porg = (double *)at(L.porg);
+  auto *frecs = (typename Pol::FarRec *)at(L.frecs);
   hipLaunchKernelGGL((sorted_pack_kernel<Pol>), dim3((n_j + 255) / 256),
                      dim3(256), 0, st, js, n_j, perm_j, sjw,
-                     (const double *)ctx, recs, origins, (TileBox *)boxes);
+                     (const double *)ctx, recs, origins, (TileBox *)boxes,
+                     porg, frecs);
   if (!same)
     hipLaunchKernelGGL((sorted_gather_kernel<Pol::kFields>),
                        dim3((n_i + 255) / 256), dim3(256), 0, st, is, n_i,
@@ -186,23 +189,23 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   hipLaunchKernelGGL((sorted_engine<Pol>), dim3((unsigned)waves, (unsigned)SB),
                      dim3(kSBlock), 0, st, sir, n_i, sjr,
                      (const typename Pol::JRec *)recs, n_j, boxes, jchunk, part,
-                     (const double *)origins);
+                     (const double *)origins,
+                     (const typename Pol::FarRec *)frecs,
+                     (const double *)porg);
   if (timed) (void)hipEventRecord(ev1, st);
   hipLaunchKernelGGL((sorted_finish_kernel<Pol>), dim3((n_i + 255) / 256),
                      dim3(256), 0, st, (const double *)part, SB, n_i, perm_i,
                      out, flag_tau() * flag_tau(), flags);
   double *rpart = (double *)at(L.rpart);
   {
-    // nseg groups of kWaves chunks of cw j's
-    const int cw = (n_j + kWaves * nseg - 1) / (kWaves * nseg);
-    hipLaunchKernelGGL((sorted_recompute_kernel<P64>),
-                       dim3((kRecomputeRows / 64) * nseg), dim3(kBlock), 0,
-                       st, sir, sjr, n_j, (const int *)flags, cw, nseg,
-                       kRecomputeRows, rpart);
+    // at most nseg groups of kWaves chunks per row-wave (recompute_shape)
+    const int rblocks = (kRecomputeRows / 64) * nseg;
+    hipLaunchKernelGGL((sorted_recompute_kernel<P64>), dim3(rblocks),
+                       dim3(kBlock), 0, st, sir, sjr, n_j, (const int *)flags,
+                       nseg, rpart);
     hipLaunchKernelGGL((sorted_recompute_finish<P64>), dim3(kRecomputeBlocks),
                        dim3(256), 0, st, sir, sjr, n_j, (const int *)flags,
-                       nseg, kRecomputeRows, (const double *)rpart, perm_i,
-                       out);
+                       rblocks, nseg, (const double *)rpart, perm_i, out);
     set_last_flags(flags, st);
   }
   const hipError_t e = hipGetLastError();

@@ -67,7 +67,9 @@ constexpr int kSctxVals = 14;                  // per-block context values
 // unflagged row stays below 0.7 * 8.7e-7 = 6.1e-7 (3.7e-7 measured); 0.3-0.6%
 // of the rows are flagged at the default near field (c = 5).
 constexpr double kFlagTau = 0.7;
-constexpr int kRecomputeRows = 1024;  // flagged rows recomputed in chunks
+// The FP64 recomputation runs (kRecomputeRows / 64) * recompute_groups(nj)
+// blocks; recompute_shape spreads the flagged rows over them.
+constexpr int kRecomputeRows = 1024;
 constexpr int kRecomputeMinJ = 64;    // j's per recompute chunk (minimum)
 constexpr int kRecomputeGroups = 256; // chunk groups of 4 (one per block)
 constexpr int kRecomputeBlocks = 1024;  // sorted_recompute_finish
@@ -78,6 +80,28 @@ constexpr int kRecomputeBlocks = 1024;  // sorted_recompute_finish
 inline int recompute_groups(int nj) {
   int n = (nj + 4 * kRecomputeMinJ - 1) / (4 * kRecomputeMinJ);
   return n < 1 ? 1 : (n > kRecomputeGroups ? kRecomputeGroups : n);
+}
+
+// How the FP64 recomputation of `count` flagged rows uses its `blocks`
+// blocks: `rows` rows (a multiple of 64 at most, count beyond 64 * blocks
+// is left to sorted_recompute_finish) in row-waves of 64, each row-wave
+// over `groups` chunk groups (at most max_groups, so that a chunk keeps at
+// least kRecomputeMinJ j's).  Few flagged rows get many short chunks, many
+// rows fewer and longer ones; rows * groups <= 64 * blocks always.  Both
+// recompute kernels derive it from the same count.
+struct RecomputeShape {
+  int rows, groups;
+};
+__host__ __device__ inline RecomputeShape recompute_shape(int count,
+                                                          int blocks,
+                                                          int max_groups) {
+  int rw = (count + 63) / 64;
+  if (rw > blocks) rw = blocks;
+  int g = rw > 0 ? blocks / rw : 1;
+  if (g > max_groups) g = max_groups;
+  if (g < 1) g = 1;
+  const int rows = count < rw * 64 ? count : rw * 64;
+  return RecomputeShape{rows, g};
 }
 
 // Geometry of one radix sort: nb blocks of ch keys (ch a multiple of 256).
@@ -389,12 +413,14 @@ static __global__ __launch_bounds__(256) void radix_scatter_kernel(
 // Sorted FP64 copies of the j fields, per-tile origins and boxes, and the
 // j records, one thread per sorted position p; a tile is the 32 lanes of a
 // half wave.  The last tile's spare lanes repeat its last j, which changes
-// no minimum or maximum.
+// no minimum or maximum.  kFarRec policies also get the far-tile records,
+// positions relative to the centre of the tile's box (porg).
 template <class Pol>
 __global__ __launch_bounds__(256) void sorted_pack_kernel(
     Soa js, int nj, const int *__restrict__ perm, SoaW sj,
     const double *__restrict__ ctx, typename Pol::JRec *__restrict__ jr,
-    double *__restrict__ origins, TileBox *__restrict__ boxes) {
+    double *__restrict__ origins, TileBox *__restrict__ boxes,
+    double *__restrict__ porg, typename Pol::FarRec *__restrict__ fr) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int o = perm[min(p, nj - 1)];
   if (p < nj)
@@ -421,10 +447,17 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
   }
   bx.near2 = tile_max((double)rec.near2);
   bx.pad = 0.0;
+  double po[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (Pol::kFarRec) {
+    for (int k = 0; k < 3; ++k) po[k] = 0.5 * (bx.lo[k] + bx.hi[k]);
+    if (p < nj) fr[p] = Pol::far_rec(rec, po);
+  }
   if ((threadIdx.x & (kTile - 1)) == 0 && p < nj) {
     const int t = p / kTile;
     for (int k = 0; k < 4; ++k) origins[4 * t + k] = org[k];
     boxes[t] = bx;
+    if constexpr (Pol::kFarRec)
+      for (int k = 0; k < 4; ++k) porg[4 * t + k] = po[k];
   }
 }
 
@@ -447,14 +480,16 @@ __global__ __launch_bounds__(256) void sorted_gather_kernel(
 constexpr int kSWaves = 4;
 constexpr int kSBlock = 64 * kSWaves;
 
-// 72 VGPRs (7 waves per SIMD; a few per-tile values spill to scratch
-// outside the pair loops): measured faster than the 82 registers and 5
-// waves the compiler picks by itself.
+// Pol::kWavesPerEu waves per SIMD at least: for AccJerk 7 (72 VGPRs; a few
+// values spill to scratch outside the pair loops), measured faster than the
+// 82 registers and 5 waves the compiler picks by itself.
 template <class Pol>
-__global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(7, 8))) void sorted_engine(
+__global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kWavesPerEu, 8))) void sorted_engine(
     Soa si, int ni, Soa sj, const typename Pol::JRec *__restrict__ jr, int nj,
     const TileBox *__restrict__ boxes, int jchunk,
-    double *__restrict__ part, const double *__restrict__ aux) {
+    double *__restrict__ part, const double *__restrict__ aux,
+    const typename Pol::FarRec *__restrict__ fr,
+    const double *__restrict__ porg) {
   constexpr int kPart = Pol::kPart;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -506,25 +541,45 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
     typename Pol::A32 a;
     Pol::zero(a);
     Pol::tile_begin(q, aux, j0 / kTile);
-    typename Pol::JRec r = jr[j0];
     if ((farbits >> (tt & 63)) & 1) {
-      if (nk == kTile) {
+      if constexpr (Pol::kFarRec) {
+        Pol::tile_begin_far(q, porg + 4 * (j0 / kTile));
+        typename Pol::FarRec r = fr[j0];
+        if (nk == kTile) {
 #pragma unroll 2
-        for (int k = 0; k < kTile; ++k) {
-          const typename Pol::JRec rn = jr[j0 + min(k + 1, kTile - 1)];
-          Pol::pair32_far(q, r, a);
-          r = rn;
+          for (int k = 0; k < kTile; ++k) {
+            const typename Pol::FarRec rn = fr[j0 + min(k + 1, kTile - 1)];
+            Pol::pair32_farrec(q, r, a);
+            r = rn;
+          }
+        } else {
+          for (int k = 0; k < nk; ++k) {
+            const typename Pol::FarRec rn = fr[j0 + min(k + 1, nk - 1)];
+            Pol::pair32_farrec(q, r, a);
+            r = rn;
+          }
         }
       } else {
-        for (int k = 0; k < nk; ++k) {
-          const typename Pol::JRec rn = jr[j0 + min(k + 1, nk - 1)];
-          Pol::pair32_far(q, r, a);
-          r = rn;
+        typename Pol::JRec r = jr[j0];
+        if (nk == kTile) {
+#pragma unroll 2
+          for (int k = 0; k < kTile; ++k) {
+            const typename Pol::JRec rn = jr[j0 + min(k + 1, kTile - 1)];
+            Pol::pair32_far(q, r, a);
+            r = rn;
+          }
+        } else {
+          for (int k = 0; k < nk; ++k) {
+            const typename Pol::JRec rn = jr[j0 + min(k + 1, nk - 1)];
+            Pol::pair32_far(q, r, a);
+            r = rn;
+          }
         }
       }
       Pol::fold(a, acc);
       Pol::stat(a, sig);
     } else {
+      typename Pol::JRec r = jr[j0];
       uint32_t near = 0u;
       if (nk == kTile) {
 #pragma unroll 2
@@ -607,44 +662,45 @@ __global__ __launch_bounds__(256) void sorted_finish_kernel(
 
 // The flagged rows in FP64 (P64, the FP64 policy of the same kernel), in
 // the pair engine's shape: a wave holds 64 flagged rows (one per lane) and
-// walks one chunk of cw j's, so the j data is read once per 64 rows.  The
+// walks one chunk of j's, so the j data is read once per 64 rows.  The
 // chunk passes through LDS 64 records at a time (one coalesced load per
 // lane, then wave-uniform LDS reads), so a wave waits for memory once per
 // 64 pairs rather than once per pair.  A block's 4 waves take the same 64
-// rows and 4 consecutive chunks and add their sums in LDS in wave order;
-// block b = (row-wave b / ngroups, chunk group b % ngroups) writes
-// rpart[(row * ngroups + group) * kOut + k] for the first min(count, cap)
-// flagged rows.  Waves without rows still take part in the barriers.
+// rows and 4 consecutive chunks and add their sums in LDS in wave order.
+// With sh = recompute_shape(flag count, gridDim.x, max_groups), block
+// b = (row-wave b / sh.groups, chunk group b % sh.groups) writes
+// rpart[(row * sh.groups + group) * kOut + k] for the first sh.rows flagged
+// rows; blocks beyond those row-waves return at once.
 template <class P64>
 __global__ __launch_bounds__(kBlock) void sorted_recompute_kernel(
-    Soa si, Soa sj, int nj, const int *__restrict__ flags, int cw,
-    int ngroups, int cap, double *__restrict__ rpart) {
+    Soa si, Soa sj, int nj, const int *__restrict__ flags, int max_groups,
+    double *__restrict__ rpart) {
   constexpr int K = P64::kOut;
   __shared__ typename P64::JRec jbuf[kWaves][64];
   __shared__ double red[kWaves - 1][K][64];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int count = min(flags[0], cap);
-  const int rw = blockIdx.x / ngroups, grp = blockIdx.x - rw * ngroups;
-  const bool active = rw * 64 < count;  // block-uniform
+  const RecomputeShape sh =
+      recompute_shape(flags[0], (int)gridDim.x, max_groups);
+  const int rw = blockIdx.x / sh.groups, grp = blockIdx.x - rw * sh.groups;
+  if (rw * 64 >= sh.rows) return;  // block-uniform, before any barrier
+  const int count = sh.rows;
   const int e = rw * 64 + lane;
-  const int p = active ? flags[1 + min(e, count - 1)] : 0;
+  const int p = flags[1 + min(e, count - 1)];
   const typename P64::Params prm{};
   const typename P64::I q = P64::load_i(si, p, prm);
   typename P64::A64 a;
   P64::zero(a);
+  const int cw = (nj + kWaves * sh.groups - 1) / (kWaves * sh.groups);
   const int jb = (grp * kWaves + w) * cw, je = min(nj, jb + cw);
   for (int j0 = jb; j0 < jb + cw; j0 += 64) {
-    if (active && j0 + lane < je) jbuf[w][lane] = P64::pack(sj, j0 + lane, 0.0);
+    if (j0 + lane < je) jbuf[w][lane] = P64::pack(sj, j0 + lane, 0.0);
     __syncthreads();
-    if (active) {
-      const int nk = min(64, je - j0);
+    const int nk = min(64, je - j0);
 #pragma unroll 4
-      for (int k = 0; k < nk; ++k) P64::pair(q, jbuf[w][k], a, prm);
-    }
+    for (int k = 0; k < nk; ++k) P64::pair(q, jbuf[w][k], a, prm);
     __syncthreads();
   }
-  if (!active) return;  // block-uniform, after the last barrier
   double v[K];
   P64::values(a, v);
   if (w > 0) {
@@ -659,33 +715,35 @@ __global__ __launch_bounds__(kBlock) void sorted_recompute_kernel(
   }
 #pragma unroll
   for (int k = 0; k < K; ++k)
-    rpart[((size_t)e * ngroups + grp) * K + k] = v[k];
+    rpart[((size_t)e * sh.groups + grp) * K + k] = v[k];
 }
 
-// Outputs of the flagged rows, one block per row (grid-stride): rows below
-// cap add their ngroups partials (thread t the groups t, t + 256, ... in
-// order, then a fixed LDS tree); rows beyond cap, if any, are evaluated
-// whole here, the block's threads over the j's.
+// Outputs of the flagged rows, one block per row (grid-stride): the first
+// sh.rows rows (sh as in sorted_recompute_kernel, which ran on
+// `blocks` blocks) add their sh.groups partials (thread t the groups t,
+// t + 256, ... in order, then a fixed LDS tree); rows beyond, if any, are
+// evaluated whole here, the block's threads over the j's.
 template <class P64>
 __global__ __launch_bounds__(256) void sorted_recompute_finish(
-    Soa si, Soa sj, int nj, const int *__restrict__ flags, int ngroups,
-    int cap, const double *__restrict__ rpart,
+    Soa si, Soa sj, int nj, const int *__restrict__ flags, int blocks,
+    int max_groups, const double *__restrict__ rpart,
     const int *__restrict__ perm_i, Out out) {
   constexpr int K = P64::kOut;
   __shared__ double red[K][256];
   const int t = threadIdx.x;
   const int count = flags[0];
+  const RecomputeShape sh = recompute_shape(count, blocks, max_groups);
   const typename P64::Params prm{};
   for (int e = blockIdx.x; e < count; e += gridDim.x) {
     const int p = flags[1 + e];
     double v[K];
-    if (e < cap) {
+    if (e < sh.rows) {
 #pragma unroll
       for (int k = 0; k < K; ++k) v[k] = 0.0;
-      for (int c = t; c < ngroups; c += 256) {
+      for (int c = t; c < sh.groups; c += 256) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          v[k] += rpart[((size_t)e * ngroups + c) * K + k];
+          v[k] += rpart[((size_t)e * sh.groups + c) * K + k];
       }
     } else {
       const typename P64::I q = P64::load_i(si, p, prm);
@@ -719,7 +777,7 @@ __global__ __launch_bounds__(256) void sorted_recompute_finish(
 // count.
 struct SortedLayout {
   size_t ctxpart, ctx, keys[2][2], vals[2][2], hist[2][kSortPasses], flags,
-      sj, si, origins, boxes, recs, rpart, total;
+      sj, si, origins, boxes, recs, porg, frecs, rpart, total;
   size_t zero, nzero;
 };
 
@@ -763,6 +821,9 @@ SortedLayout sorted_layout(int ni, int nj, bool same, SortGeom gj,
   L.origins = take(sizeof(double) * 4 * (size_t)ntiles);
   L.boxes = take(sizeof(TileBox) * (size_t)ntiles);
   L.recs = take(sizeof(typename Pol::JRec) * (size_t)nj);
+  // far-tile records and the tiles' position origins (kFarRec policies)
+  L.porg = take(Pol::kFarRec ? sizeof(double) * 4 * (size_t)ntiles : 0);
+  L.frecs = take(Pol::kFarRec ? sizeof(typename Pol::FarRec) * (size_t)nj : 0);
   L.rpart = take(sizeof(double) * (size_t)kRecomputeRows * nseg * Pol::kOut);
   // nseg: chunk groups of the FP64 recomputation
   L.total = o;
