@@ -50,6 +50,7 @@ METRIC = ("Acc+Jrk Ginteractions/s at N=64K, FP32-comp vs FP64 at <=1e-6 "
           "rel err; %peak")
 FLOPS_PER_PAIR = 42
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 
 
 def parse():
@@ -434,9 +435,12 @@ def bench_pn(args, torch):
     closure of momentum and angular momentum with their PN accumulators, and
     pair throughput against the reference's C kernels for the same pair
     visits (Acc and PNAcc, and Sakura over every pair as the reference
-    applies it) on a bounded sample of host steps."""
+    applies it) on a bounded sample of host steps.  pnacc_roofline: the
+    PNAcc kernel alone (order 5, PNACC_FLOPS[5] flop per pair) against the
+    FP64 vector peak, on the final state and on the N = 65536 benchmark
+    model."""
+    from tupan_amd import device, kernels, spec
     from tupan_amd import integrator as ig
-    from tupan_amd import kernels
     from tupan_amd.ics import UniformMass, make_plummer, scale_radius
     n, dt, t_end, rcut, c = 2048, 1.0 / 8192, 0.1, 1e-3, 1e3
     phi = lambda p: kernels.compute("phi", p, p, mode="fp64")[0]
@@ -491,6 +495,23 @@ def bench_pn(args, torch):
         o.call("sakura_kernel", h, h, dt / 2, 1)
         reps += 1
     cpu_rate = 3.0 * n * n * reps / (time.perf_counter() - t)
+    # the PNAcc kernel alone against the FP64 vector peak: on the final
+    # state, and on the N = 65536 Plummer model with the same c and order
+    roof = {}
+    for tag, hs in (("n2048_final_state", h), ("n65536_plummer", None)):
+        if hs is None:
+            hs = make_workload(65536)
+        dd = device.to_device(hs, spec.F8)
+        nn = dd["mass"].numel()
+        out = tuple(torch.empty(nn, dtype=torch.float64, device="cuda")
+                    for _ in range(3))
+        step = lambda: device.compute("pnacc", dd, dd, 5, inv, out=out)
+        _, k_ms, _ = time_steps(step, 20 if nn <= 4096 else 5, 2, torch)
+        ach = spec.PNACC_FLOPS[5] * float(nn) * nn / (k_ms * 1e-3) / 1e12
+        roof[tag] = {"bound": "valu", "achieved": ach,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP64_TFLOPS, "kernel_ms": k_ms,
+                     "flops_per_interaction": spec.PNACC_FLOPS[5]}
     result = {
         "metric": "PN cluster pair visits/s (far Acc + PNAcc + near Sakura),"
                   " N=2048",
@@ -502,6 +523,7 @@ def bench_pn(args, torch):
                                " r_h=0.1, eps=0, pn_order 5, c=1e3, Sakura "
                                "pairs r<1e-3, dt=1/8192, t=0.1",
                    "close_pairs_t0": close0 // 2, "graph": bool(args.graph)},
+        "pnacc_roofline": roof,
         "energy_rel_err": abs(E1 - E0) / abs(E0),
         "momentum_closure": float(np.abs(P1 - P0).max()),
         "angular_momentum_closure": float(np.abs(L1 - L0).max()),

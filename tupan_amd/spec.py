@@ -7,8 +7,17 @@ Tstep :382, PNAcc :434, Sakura :501-502, NREG_X :555-557, NREG_V :609-610).
 
 ``flops`` is the reference's own per-interaction count from the comments
 of each *_kernel_common.h (phi :32, acc :38, acc_jerk :56, snap_crackle :95,
-tstep :60); PNAcc and Sakura have none ("40+???", "36 + ?").
+tstep :60).  PNAcc has none ("40+???", pnacc_kernel_common.h:70): its count
+here is PNACC_FLOPS[pn_order], the operations one pair executes (each +, -,
+*, / once; negations, rational constants and additions of unused PN slots
+free), counted by tests/pnacc_flops.cpp on pn_terms.inc plus the 33 of the
+set-up and accumulation lines; the "pnacc" entry carries the order-5 count
+(1PN + 2PN + 2.5PN).  Sakura has none ("36 + ?") and none is defined: its
+cost per pair depends on the Kepler solve's iterations.
 """
+
+# floating-point operations of one PNAcc pair per PN order
+PNACC_FLOPS = {0: 50, 1: 50, 2: 82, 3: 82, 4: 179, 5: 197, 6: 452, 7: 626}
 from collections import namedtuple
 
 F5 = ("mass", "rx", "ry", "rz", "eps2")
@@ -25,7 +34,7 @@ KERNELS = {
     "snap_crackle": Kernel("snap_crackle", 3, F14,
                            ("sx", "sy", "sz", "cx", "cy", "cz"), True, 114),
     "tstep": Kernel("tstep", 4, F8, ("tstep", "tstepij"), True, 42),
-    "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), True, None),
+    "pnacc": Kernel("pnacc", 5, F8, ("pnax", "pnay", "pnaz"), True, 197),
     "sakura": Kernel("sakura", 6, F8,
                      ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False, None),
     "nreg_x": Kernel("nreg_x", 9, F8,
