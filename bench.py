@@ -68,6 +68,9 @@ def parse():
                    help="pn and nreg workloads: replay each step as a HIP "
                         "graph")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-full", action="store_true",
+                   help="ensemble workload: skip the run of every cluster "
+                        "to t = 1")
     p.add_argument("--sizes", default="16384,65536,262144",
                    help="sizes workload: the N values (comma-separated)")
     p.add_argument("--workload", default="accjerk",
@@ -258,7 +261,9 @@ def bench_ensemble(args, torch):
     = 4 N_c^2 pair interactions.  Timed: `steps` ensemble steps after the
     autotuner and `warmup` steps.  Baseline: the single-system device path
     (tupan_amd.integrator, one cluster at a time, kernel launches from the
-    host) on every 16th cluster for the same number of steps."""
+    host) on every 16th cluster for the same number of steps.  full_run_t1
+    (unless --no-full): every cluster to t = 1 in both modes
+    (ensemble_full_run)."""
     from tupan_amd import ensemble, kernels
     from tupan_amd import integrator as ig
     sizes = ensemble.cluster_sizes(512, seed=2)
@@ -318,6 +323,8 @@ def bench_ensemble(args, torch):
         t_b += time.perf_counter() - t
         pairs_b += 4.0 * float(sizes[c])**2 * args.steps
     loop_rate = pairs_b / t_b
+    full = ensemble_full_run(args, sizes, fields, best, best32, torch) \
+        if not args.no_full else None
     result = {
         "metric": "ensemble AccJrk+Tstep Ginteractions/s, 512 clusters, "
                   "adaptive Hermite-4",
@@ -340,8 +347,73 @@ def bench_ensemble(args, torch):
                              "unit": "Ginteractions/s",
                              "clusters": len(sample),
                              "speedup": ens_rate / loop_rate},
+        "full_run_t1": full,
     }
     print(json.dumps(result))
+
+
+def ensemble_full_run(args, sizes, fields, best, best32, torch):
+    """Every cluster integrated to t_end = 1 (the configured length), in
+    FP64 and with FP32C pair passes, from the same start and with each
+    mode's autotuned launch shape: wall time, ensemble steps, pair
+    interactions, per-cluster energy error |E(1) - E(0)| / |E(0)| (Phi on
+    the GPU in FP64), and the FP32C end state against FP64's."""
+    from tupan_amd import ensemble, kernels
+    off = np.concatenate([[0], np.cumsum(sizes)])
+
+    def energies(st):
+        e = np.empty(len(sizes))
+        for c in range(len(sizes)):
+            sl = slice(off[c], off[c + 1])
+            d = {f: np.ascontiguousarray(st[f][sl]) if f in st
+                 else np.ascontiguousarray(fields[f][sl])
+                 for f in ensemble.BASE}
+            phi = kernels.compute("phi", d, d, mode="fp64")[0]
+            m = d["mass"]
+            kin = 0.5 * (m * (d["vx"]**2 + d["vy"]**2 + d["vz"]**2)).sum()
+            e[c] = kin + 0.5 * (m * phi).sum()
+        return e
+
+    e0 = energies({f: fields[f] for f in ensemble.STATE})
+    out = {}
+    for mode, shape in (("fp64", best), ("fp32c", best32)):
+        ens = ensemble.Ensemble(sizes, 0.02, 1.0, mode=mode)
+        ens.set_launch(*shape)
+        ens.set_state(fields)
+        torch.cuda.synchronize()
+
+        def progress(k, left, pairs):
+            if k % 512 == 0:
+                print("[bench] full run %s: %d steps, %d clusters left"
+                      % (mode, k, left), file=sys.stderr, flush=True)
+
+        stats = ens.run(batch=64, graph=True, progress=progress)
+        st, tc, ns = ens.get_state()
+        ens.close()
+        de = np.abs(energies(st) - e0) / np.abs(e0)
+        out[mode] = {"seconds": stats["seconds"], "ensemble_steps":
+                     stats["steps"], "pairs": stats["pairs"],
+                     "value": stats["pairs"] / stats["seconds"] / 1e9,
+                     "unit": "Ginteractions/s",
+                     "cluster_steps_total": int(ns.sum()),
+                     "all_at_t1": bool(np.all(np.abs(tc - 1.0) < 1e-12)),
+                     "energy_rel_err_max": float(de.max()),
+                     "energy_rel_err_median": float(np.median(de)),
+                     "_state": st, "_ns": ns}
+    a, b = out["fp32c"].pop("_state"), out["fp64"].pop("_state")
+    na, nb = out["fp32c"].pop("_ns"), out["fp64"].pop("_ns")
+    same = na == nb
+    err = [max(np.abs(a[f][off[c]:off[c + 1]] - b[f][off[c]:off[c + 1]]).max()
+               / np.abs(b[f][off[c]:off[c + 1]]).max()
+               for f in ensemble.STATE) for c in range(len(sizes))]
+    err = np.array(err)
+    out["fp32c_vs_fp64"] = {
+        "speedup": out["fp64"]["seconds"] / out["fp32c"]["seconds"],
+        "clusters_same_step_count": int(same.sum()),
+        "state_scale_err_max_same_steps": float(err[same].max())
+        if same.any() else None,
+        "state_scale_err_median": float(np.median(err))}
+    return out
 
 
 def bench_sizes(args, torch):

@@ -100,7 +100,7 @@ int tupan_set_timing(int on);
  * r^2 < (m_j/<m>) (c nj^(-1/3))^2 are evaluated in FP64 (a threshold on
  * the pair force, c mean interparticle spacings of a unit-size system for
  * a mean-mass j).  c = 0 leaves only the r^2 = 0 pairs; default 5.  (The
- * ensemble's FP32C passes use their own factor, 2.) */
+ * ensemble's FP32C passes use their own factor, 0.5.) */
 int tupan_set_near_field(double c);
 /* FP32C calls with nj > 4096 run over Morton-sorted copies of the
  * particles: tiles whose bounding box lies beyond the near field skip the
@@ -339,10 +339,12 @@ int tupan_phi_adjoint_dev(int mode, size_t ni, const double *const *ip,
  * step (Tstep and AccJerk at the start state in one pair pass, block-step
  * choice, predict, 2 x (AccJerk + correct), clock) is captured once into a
  * HIP graph and replayed.  FP64 arithmetic by default;
- * tupan_ensemble_set_mode(TUPAN_MODE_FP32C) evaluates the two corrector
- * AccJerk sums in the FP32C arithmetic of the single-system kernels
- * (near-field factor and velocity tile origins per cluster); the start
- * pass (Tstep with AccJerk) and the updates stay FP64.  Device memory is
+ * tupan_ensemble_set_mode(TUPAN_MODE_FP32C) evaluates the three pair
+ * passes (Tstep with AccJerk at the start state, the two corrector AccJerk
+ * sums) in the FP32C arithmetic of the single-system kernels (FP32 tile
+ * sums folded into FP64, FP32 offsets to per-tile FP64 origins, near pairs
+ * in FP64 with a near-field factor per cluster); the updates and the
+ * block-step choice stay FP64.  Device memory is
  * allocated at create and at the first step after set_launch or set_mode;
  * steps allocate nothing. */
 typedef struct tupan_ensemble tupan_ensemble;
@@ -354,8 +356,8 @@ int tupan_ensemble_destroy(tupan_ensemble *e);
  * item, resident waves per SIMD of the persistent kernels. */
 int tupan_ensemble_set_launch(tupan_ensemble *e, int ipl, int jchunk,
                               int waves_per_simd);
-/* mode = TUPAN_MODE_FP64 (default) or TUPAN_MODE_FP32C (the corrector
- * AccJerk passes; one i-particle per lane there, jchunk a multiple of 32). */
+/* mode = TUPAN_MODE_FP64 (default) or TUPAN_MODE_FP32C (the pair passes;
+ * one i-particle per lane there, jchunk a multiple of 32). */
 int tupan_ensemble_set_mode(tupan_ensemble *e, int mode);
 /* Storage order of each cluster's particles on the device, applied at the
  * next set_state (get_state returns input order either way):

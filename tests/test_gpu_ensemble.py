@@ -9,8 +9,9 @@ component's scale (the pair sums are split differently, so only rounding
 differs).  The captured-graph replay and the launched-kernel path give
 identical bits, and the autotuner's launch shapes agree to 1e-11.
 
-FP32C mode (corrector AccJerk sums in FP32C arithmetic): every cluster takes the
-same steps as in FP64 and ends within 1e-7 of each component's scale; the
+FP32C mode (the Tstep + AccJerk start pass and the corrector AccJerk sums in
+FP32C arithmetic): every cluster takes the same steps as in FP64 and ends
+within 1e-7 of each component's scale; the
 graph replay is bitwise the launched kernels; j-chunk lengths change only
 the FP64 grouping of the tile sums (1e-11); bad modes and j chunks that are
 not whole tiles are refused.  The device storage order (Morton or input

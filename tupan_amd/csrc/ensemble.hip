@@ -21,11 +21,12 @@
 // Pair sums: a persistent kernel walks a static list of work items
 // (cluster, i-wave, j-chunk), sorted by cost and dealt round-robin to the
 // resident waves, so that 64-particle and 4096-particle clusters share the
-// chip evenly.  Each item writes FP64 partial sums; a finishing kernel adds a
+// chip evenly.  (Taking items from an atomic counter instead was measured
+// 15% slower: the waves then wait for the counter before every item.)  Each item writes FP64 partial sums; a finishing kernel adds a
 // cluster's chunks in chunk order (bit-reproducible).  The pair arithmetic
 // is the single-evaluation FP64 policies' (gravity.hip); the Tstep and
-// AccJerk sums at the start state share one pass (TSAJ), and in FP32C mode
-// the two corrector AccJerk passes use the FP32C policy (accjerk.hpp).
+// AccJerk sums at the start state share one pass (TSAJ).  In FP32C mode
+// the three pair passes run in FP32C arithmetic (Tsaj32, AJ32E below).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -276,91 +277,306 @@ __global__ __launch_bounds__(kThreads) void ens_pack(int ntot, Fields s,
 }
 
 // ---------------------------------------------------------------- FP32C
-// AccJerk in the FP32C arithmetic of the single-system kernels (f32c.hpp,
-// accjerk.hpp) without the Newton refinement of 1/sqrt: FP64 separations
-// rounded once to FP32, FP32 tile sums folded into FP64, velocity offsets
-// to per-tile FP64 origins, and near pairs (below the j record's
-// threshold) re-evaluated in FP64 from the slab.  Tiles are
-// numbered within each cluster (tbase: the cluster's first tile), so a work
-// item's j range -- a multiple of the tile length from the cluster's start --
-// covers whole tiles.  The near-field threshold uses each cluster's own
-// factor and largest e2 (ClusterCtx, set with the state).
+// The pair passes in the FP32C arithmetic of the single-system kernels
+// (f32c.hpp, accjerk.hpp) without the Newton refinement of 1/sqrt and
+// without the debiasing factors: FP32 tile sums folded into FP64,
+// velocity offsets to per-tile FP64 origins, and near pairs (below the j
+// record's threshold) re-evaluated in FP64 from the slab.  As in the
+// sorted single-system engine, each (wave, tile) pair of boxes is tested
+// first: a tile whose box is beyond every near threshold of its j's from
+// the box of the wave's 64 i's runs the untested loop on far records, the
+// others the tested loop.  Both take the positions as FP32 offsets to the
+// tile's box centre (the j offset rounded once when packed, the i offset
+// once per tile); in a near tile a pair's offsets can be larger than its
+// separation, which costs the pair sums accuracy (about 1e-6 of a pair
+// term at the near-field radius of a whole-cluster tile) that the
+// ensemble's states do not see (below).  Tiles are numbered within each cluster
+// (tbase: the cluster's first tile), so a work item's j range -- a
+// multiple of the tile length from the cluster's start -- covers whole
+// tiles.  The near-field threshold uses each cluster's own factor and
+// largest e2 (ClusterCtx, set with the state).
 using AJ32 = AccJerkF32cTile;
 // Near-field factor of the ensemble's FP32C passes (the single-system
-// kernels' rule with c = 2): clusters of 64-4096 particles in Morton order
-// keep their near pairs to a few per cent of the j's at this radius.
-constexpr double kEnsNearC = 2.0;
+// kernels' rule with c = 0.5: radius 0.5 N_c^(-1/3) for equal masses).
+// The ensemble needs its states, not each pair sum, close to FP64's: on
+// the 512 Plummer clusters (N_c 64-4096) every cluster stays within 6e-9
+// of FP64 after 6 steps.  (Measured with FP64 separations in the tested
+// loop: 3e-10, 6e-10 and 1.2e-9 at c = 2, 0.5 and 0.25, and c = 0.5 12%
+// faster than c = 2.)
+constexpr double kEnsNearC = 0.5;
 
 struct ClusterCtx {
   double factor, e2max;
   int tbase, pad;
 };
 
-// Velocity origin of every tile of the active clusters: the midpoint of the
-// tile's velocity range (tile_origin_kernel's rule), one thread per tile.
-__global__ __launch_bounds__(kThreads) void ens_origins(
+// Per tile of the active clusters, one thread per tile: the velocity
+// origin (midpoint of the tile's velocity range, tile_origin_kernel's
+// rule), the bounding box of the positions with the largest near-field
+// threshold of the tile's j's, and the box centre (the far records'
+// position origin).
+__global__ __launch_bounds__(kThreads) void ens_tiles(
     int ntiles, const int *__restrict__ tcid, const ClusterInfo *__restrict__ cl,
     const ClusterCtx *__restrict__ cx, const int *__restrict__ active,
-    Fields s, double *__restrict__ orig) {
+    Fields s, double *__restrict__ orig, TileBox *__restrict__ boxes,
+    double *__restrict__ porg) {
   for (int g = blockIdx.x * kThreads + threadIdx.x; g < ntiles;
        g += gridDim.x * kThreads) {
     const int c = tcid[g];
     if (!active[c]) continue;
     const ClusterInfo ci = cl[c];
-    const int j0 = ci.off + (g - cx[c].tbase) * kTile;
+    const ClusterCtx x = cx[c];
+    const PackCtx pc{x.factor, x.e2max, nullptr};
+    const int j0 = ci.off + (g - x.tbase) * kTile;
     const int je = min(ci.off + ci.n, j0 + kTile);
+    TileBox bx;
     for (int k = 0; k < 3; ++k) {
       const double *v = s.f[FVX + k];
-      double lo = v[j0], hi = lo;
+      const double *r = s.f[FX + k];
+      double lo = v[j0], hi = lo, plo = r[j0], phi = plo;
       for (int j = j0 + 1; j < je; ++j) {
         lo = fmin(lo, v[j]);
         hi = fmax(hi, v[j]);
+        plo = fmin(plo, r[j]);
+        phi = fmax(phi, r[j]);
       }
       orig[4 * g + k] = 0.5 * (lo + hi);
+      bx.lo[k] = plo;
+      bx.hi[k] = phi;
+      porg[4 * g + k] = 0.5 * (plo + phi);
     }
+    double n2 = 0.0;
+    for (int j = j0; j < je; ++j)
+      n2 = fmax(n2, (double)near_threshold(s.f[FM][j], s.f[FE2][j], pc));
+    bx.near2 = n2;
+    bx.pad = 0.0;
+    boxes[g] = bx;
+    orig[4 * g + 3] = 0.0;
+    porg[4 * g + 3] = 0.0;
   }
 }
 
-// FP32C j records (AJ32::pack with the cluster's tile origin and
-// near-field context).
+// j record of the tested (near-tile) loop: AJ32's far record plus the
+// near-field threshold.
+struct alignas(16) EnsNearRec {
+  float x, y, z, dvx, dvy, dvz, m, e2, near2, pad[3];
+};
+
+// FP32C j records of the tested and untested loops, with the cluster's
+// tile origins and near-field context.
 __global__ __launch_bounds__(kThreads) void ens_pack32(
     int ntot, Fields s, const int *__restrict__ cid,
     const ClusterInfo *__restrict__ cl, const ClusterCtx *__restrict__ cx,
     const int *__restrict__ active, const double *__restrict__ orig,
-    AJ32::JRec *__restrict__ rec) {
+    const double *__restrict__ porg, EnsNearRec *__restrict__ rec,
+    AJ32::FarRec *__restrict__ frec) {
   for (int i = blockIdx.x * kThreads + threadIdx.x; i < ntot;
        i += gridDim.x * kThreads) {
     const int c = cid[i];
     if (!active[c]) continue;
     const ClusterCtx x = cx[c];
-    const double *o = orig + 4 * (x.tbase + (i - cl[c].off) / kTile);
+    const int t = x.tbase + (i - cl[c].off) / kTile;
+    const double *o = orig + 4 * t;
     const PackCtx pc{x.factor, x.e2max, nullptr};
-    rec[i] = {s.f[FX][i],
-              s.f[FY][i],
-              s.f[FZ][i],
-              dsub32(s.f[FVX][i], o[0]),
-              dsub32(s.f[FVY][i], o[1]),
-              dsub32(s.f[FVZ][i], o[2]),
-              (float)s.f[FM][i],
-              (float)s.f[FE2][i],
-              near_threshold(s.f[FM][i], s.f[FE2][i], pc)};
+    const AJ32::JRec r = {s.f[FX][i],
+                          s.f[FY][i],
+                          s.f[FZ][i],
+                          dsub32(s.f[FVX][i], o[0]),
+                          dsub32(s.f[FVY][i], o[1]),
+                          dsub32(s.f[FVZ][i], o[2]),
+                          (float)s.f[FM][i],
+                          (float)s.f[FE2][i],
+                          near_threshold(s.f[FM][i], s.f[FE2][i], pc)};
+    const AJ32::FarRec f = AJ32::far_rec(r, porg + 4 * t);
+    frec[i] = f;
+    rec[i] = {f.x, f.y, f.z, f.dvx, f.dvy, f.dvz, f.m, f.e2, r.near2,
+              {0.0f, 0.0f, 0.0f}};
   }
 }
 
-// Persistent FP32C AccJerk over the work items (ens_pairs' scheme, one
-// i-particle per lane), writing the same FP64 chunk partials.
-__global__ __launch_bounds__(kThreads) void ens_pairs32(
-    const AJ32::JRec *__restrict__ rec, const Item *__restrict__ items,
+// Near test on the position offsets: rx.. = i offset - j offset, s^2 in
+// the FMA order of F32cBase::near_test, near when s^2 < near2 (or NaN).
+template <class I>
+__device__ __forceinline__ bool ens_near_test(const I &q, const EnsNearRec &r,
+                                              float &rx, float &ry, float &rz,
+                                              float &s2) {
+  rx = q.px - r.x;
+  ry = q.py - r.y;
+  rz = q.pz - r.z;
+  s2 = __builtin_fmaf(
+      rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+  return !(s2 >= r.near2);
+}
+
+// AccJerk (the corrector passes) in the tile kernel's terms.
+struct AJ32E {
+  using P = AJ32;
+  using JRec = EnsNearRec;
+  using FarRec = P::FarRec;
+  using I = P::I;
+  using A32 = P::A32;
+  using A64 = P::A64;
+  using Params = AJ::Params;
+  static constexpr int kOut = 6;
+  static constexpr int kWavesPerEu = 7;  // ens_tiles32's occupancy
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return P::load_i(s, i, NoParams{});
+  }
+  __device__ static void zero(A32 &a) { P::zero(a); }
+  __device__ static void zero(A64 &a) { P::zero(a); }
+  __device__ static void tile_begin(I &q, const double *orig, int t) {
+    P::tile_begin(q, orig, t);
+  }
+  __device__ static void tile_begin_far(I &q, const double *po) {
+    P::tile_begin_far(q, po);
+  }
+  __device__ static uint64_t pair32(const I &q, const JRec &r, A32 &a) {
+    float rx, ry, rz, s2;
+    const bool far = !ens_near_test(q, r, rx, ry, rz, s2);
+    P::body32(q, r, rx, ry, rz, s2, far, a);
+    return __builtin_amdgcn_ballot_w64(far);
+  }
+  __device__ static void pair32_farrec(const I &q, const FarRec &r, A32 &a) {
+    P::pair32_farrec(q, r, a);
+  }
+  __device__ static void fold(const A32 &a, A64 &b) { P::fold(a, b); }
+  __device__ static void values(const A64 &a, double *v) { P::values(a, v); }
+  __device__ static void near_pair(const I &q, const JRec &r, A64 &a,
+                                   const Soa &s, int j, const Params &) {
+    float rx, ry, rz, s2;
+    if (ens_near_test(q, r, rx, ry, rz, s2)) {
+      AccJerkF64::Params p64{};
+      AccJerkF64::pair(AccJerkF64::load_i(s, q.i, p64),
+                       AccJerkF64::pack(s, j, 0.0), a, p64);
+    }
+  }
+};
+
+// Tstep and AccJerk at the step's start state (TSAJ) in FP32C: AJ32's
+// records and AccJerk terms, and the Tstep terms of TSAJ::pair
+// (tstep_kernel_common.h:7-56) in FP32 on the same separation, velocity
+// offsets and 1/s.  The tstep sum folds into FP64 like the others; its
+// pair maximum is taken in FP32 and folded with fmax (exact).  Near pairs
+// are TSAJ::pair in FP64.  A near or self pair gives y = 0 and then
+// 0 * inf in gam; w2 is selected to 0 after, as in TSAJ::pair.
+struct Tsaj32 {
+  using P = AJ32;
+  using JRec = EnsNearRec;
+  using FarRec = P::FarRec;
+  struct I : P::I {
+    float m, eta;
+  };
+  using Params = TSAJ::Params;
+  static constexpr int kOut = 8;
+  static constexpr int kWavesPerEu = 7;  // ens_tiles32's occupancy
+  struct A32 {
+    float v[8];
+  };
+  using A64 = TSAJ::A;
+  __device__ static I load_i(const Soa &s, int i, const Params &prm) {
+    I q;
+    static_cast<P::I &>(q) = P::load_i(s, i, NoParams{});
+    q.m = (float)s.f[M][i];
+    q.eta = (float)prm.eta;
+    return q;
+  }
+  __device__ static void zero(A32 &a) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a.v[k] = 0.0f;
+  }
+  __device__ static void zero(A64 &a) { TSAJ::zero(a); }
+  __device__ static void tile_begin(I &q, const double *orig, int t) {
+    P::tile_begin(q, orig, t);
+  }
+  __device__ static void tile_begin_far(I &q, const double *po) {
+    P::tile_begin_far(q, po);
+  }
+  template <class R>  // JRec or FarRec
+  __device__ static void body32(const I &q, const R &r, float rx, float ry,
+                                float rz, float s2, bool far, A32 &a) {
+    float vx = q.dvx - r.dvx, vy = q.dvy - r.dvy, vz = q.dvz - r.dvz;
+    const float rv = rx * vx + ry * vy + rz * vz;
+    float y = rsq32(s2);
+    y = far ? y : 0.0f;
+    const float y2 = y * y;
+    {
+      const float v2 = vx * vx + vy * vy + vz * vz;
+      const float phi = (q.m + r.m) * y;
+      float w2 = y2 * (v2 + 2.0f * phi);
+      float gam = y2 * (w2 + 2.0f * y2 * phi);
+      gam *= q.eta * rsq32(w2);
+      w2 -= gam * rv;
+      w2 = far ? w2 : 0.0f;
+      a.v[0] += w2;
+      a.v[1] = fmaxf(w2, a.v[1]);
+    }
+    const float k = r.m * y * y2;
+    const float alpha = 3.0f * rv * y2;
+    vx -= alpha * rx;
+    vy -= alpha * ry;
+    vz -= alpha * rz;
+    a.v[2] -= k * rx;
+    a.v[3] -= k * ry;
+    a.v[4] -= k * rz;
+    a.v[5] -= k * vx;
+    a.v[6] -= k * vy;
+    a.v[7] -= k * vz;
+  }
+  __device__ static uint64_t pair32(const I &q, const JRec &r, A32 &a) {
+    float rx, ry, rz, s2;
+    const bool far = !ens_near_test(q, r, rx, ry, rz, s2);
+    body32(q, r, rx, ry, rz, s2, far, a);
+    return __builtin_amdgcn_ballot_w64(far);
+  }
+  __device__ static void pair32_farrec(const I &q, const FarRec &r, A32 &a) {
+    const float rx = q.px - r.x, ry = q.py - r.y, rz = q.pz - r.z;
+    const float s2 = __builtin_fmaf(
+        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+    body32(q, r, rx, ry, rz, s2, true, a);
+  }
+  __device__ static void fold(const A32 &a, A64 &b) {
+    b.v[0] += (double)a.v[0];
+    b.v[1] = fmax(b.v[1], (double)a.v[1]);
+#pragma unroll
+    for (int k = 2; k < 8; ++k) b.v[k] += (double)a.v[k];
+  }
+  __device__ static void values(const A64 &a, double *v) { TSAJ::values(a, v); }
+  __device__ static void near_pair(const I &q, const JRec &r, A64 &a,
+                                   const Soa &s, int j, const Params &prm) {
+    float rx, ry, rz, s2;
+    if (ens_near_test(q, r, rx, ry, rz, s2)) {
+      const int i = q.i;
+      const AccJerkF64Rec qi = {s.f[M][i],  s.f[RX][i], s.f[RY][i],
+                                s.f[RZ][i], s.f[E2][i], s.f[VX][i],
+                                s.f[VY][i], s.f[VZ][i]};
+      const AccJerkF64Rec rj = {s.f[M][j],  s.f[RX][j], s.f[RY][j],
+                                s.f[RZ][j], s.f[E2][j], s.f[VX][j],
+                                s.f[VY][j], s.f[VZ][j]};
+      TSAJ::pair(qi, rj, a, prm);
+    }
+  }
+};
+
+// Persistent FP32C pair kernel over the work items (ens_pairs' scheme, one
+// i-particle per lane), writing the same FP64 chunk partials.  Per 64 tiles
+// of an item, lane l tests tile l's box against the wave's box (far: the
+// squared gap reaches the tile's largest near threshold, so that every
+// pair has s^2 >= r^2 >= near2_j and r^2 > 0).
+template <class Pol>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::kWavesPerEu, 8))) void ens_tiles32(
+    const typename Pol::JRec *__restrict__ rec,
+    const typename Pol::FarRec *__restrict__ frec,
+    const TileBox *__restrict__ boxes, const double *__restrict__ orig,
+    const double *__restrict__ porg, const Item *__restrict__ items,
     int nitems, const ClusterInfo *__restrict__ cl,
     const ClusterCtx *__restrict__ cx, const int *__restrict__ active,
-    Fields s, const double *__restrict__ orig, double *__restrict__ part) {
+    Fields s, double *__restrict__ part, typename Pol::Params prm) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(
       (int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
   const int nwaves = gridDim.x * (kThreads / 64);
   Soa soa{};
   for (int k = 0; k < 8; ++k) soa.f[k] = s.f[k];
-  const NoParams np{};
   for (int it = wave; it < nitems; it += nwaves) {
     const Item w = items[it];
     if (!active[w.c]) continue;
@@ -368,47 +584,93 @@ __global__ __launch_bounds__(kThreads) void ens_pairs32(
     const int tbase = cx[w.c].tbase;
     int li = w.i0 + lane;
     li = li < ci.n ? li : ci.n - 1;
-    AJ32::I q = AJ32::load_i(soa, ci.off + li, np);
-    AJ32::A64 acc;
-    AJ32::zero(acc);
-    for (int j0 = w.j0; j0 < w.j1; j0 += kTile) {
-      const int nk = min(kTile, w.j1 - j0);
-      AJ32::tile_begin(q, orig, tbase + j0 / kTile);
-      AJ32::A32 a;
-      AJ32::zero(a);
-      uint32_t near = 0u;
-      const AJ32::JRec *jr = rec + ci.off + j0;
-      // the record of j+1 is fetched while j is computed; whole tiles
-      // unrolled so that the fetch is in flight over a full pair
-      AJ32::JRec r = jr[0];
-      if (nk == kTile) {
-#pragma unroll 2
-        for (int k = 0; k < kTile; ++k) {
-          const AJ32::JRec rn = jr[min(k + 1, kTile - 1)];
-          near |= near_bit(AJ32::pair32(q, r, a)) << k;
-          r = rn;
+    typename Pol::I q = Pol::load_i(soa, ci.off + li, prm);
+    double wlo[3], whi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double x = soa.f[RX + k][ci.off + li];
+      wlo[k] = wave_min(x);
+      whi[k] = wave_max(x);
+    }
+    typename Pol::A64 acc;
+    Pol::zero(acc);
+    uint64_t farbits = 0;
+    int tt = 0;
+    for (int j0 = w.j0; j0 < w.j1; j0 += kTile, ++tt) {
+      const int t = tbase + j0 / kTile;
+      if ((tt & 63) == 0) {
+        bool far = false;
+        if (j0 + lane * kTile < w.j1) {
+          const TileBox b = boxes[t + lane];
+          double d2 = 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const double gap =
+                fmax(fmax(b.lo[k] - whi[k], wlo[k] - b.hi[k]), 0.0);
+            d2 = __builtin_fma(gap, gap, d2);
+          }
+          far = d2 >= b.near2;
         }
-      } else {
-        for (int k = 0; k < nk; ++k) {
-          const AJ32::JRec rn = jr[min(k + 1, nk - 1)];
-          near |= near_bit(AJ32::pair32(q, r, a)) << k;
-          r = rn;
-        }
+        farbits = __builtin_amdgcn_ballot_w64(far);
       }
-      AJ32::fold(a, acc);
-      while (near) {
-        const int k = __builtin_ctz(near);
-        near &= near - 1u;
-        AJ32::near_pair(q, jr[k], acc, soa, soa, ci.off + j0 + k);
+      const int nk = min(kTile, w.j1 - j0);
+      Pol::tile_begin(q, orig, t);
+      Pol::tile_begin_far(q, porg + 4 * t);
+      typename Pol::A32 a;
+      Pol::zero(a);
+      if ((farbits >> (tt & 63)) & 1) {
+        const typename Pol::FarRec *fr = frec + ci.off + j0;
+        typename Pol::FarRec r = fr[0];
+        if (nk == kTile) {
+#pragma unroll 2
+          for (int k = 0; k < kTile; ++k) {
+            const typename Pol::FarRec rn = fr[min(k + 1, kTile - 1)];
+            Pol::pair32_farrec(q, r, a);
+            r = rn;
+          }
+        } else {
+          for (int k = 0; k < nk; ++k) {
+            const typename Pol::FarRec rn = fr[min(k + 1, nk - 1)];
+            Pol::pair32_farrec(q, r, a);
+            r = rn;
+          }
+        }
+        Pol::fold(a, acc);
+      } else {
+        uint32_t near = 0u;
+        const typename Pol::JRec *jr = rec + ci.off + j0;
+        // the record of j+1 is fetched while j is computed; whole tiles
+        // unrolled so that the fetch is in flight over a full pair
+        typename Pol::JRec r = jr[0];
+        if (nk == kTile) {
+#pragma unroll 2
+          for (int k = 0; k < kTile; ++k) {
+            const typename Pol::JRec rn = jr[min(k + 1, kTile - 1)];
+            near |= near_bit(Pol::pair32(q, r, a)) << k;
+            r = rn;
+          }
+        } else {
+          for (int k = 0; k < nk; ++k) {
+            const typename Pol::JRec rn = jr[min(k + 1, nk - 1)];
+            near |= near_bit(Pol::pair32(q, r, a)) << k;
+            r = rn;
+          }
+        }
+        Pol::fold(a, acc);
+        while (near) {
+          const int k = __builtin_ctz(near);
+          near &= near - 1u;
+          Pol::near_pair(q, jr[k], acc, soa, ci.off + j0 + k, prm);
+        }
       }
     }
     if (w.i0 + lane < ci.n) {
-      double v[6];
-      AJ32::values(acc, v);
+      double v[Pol::kOut];
+      Pol::values(acc, v);
 #pragma unroll
-      for (int k = 0; k < 6; ++k)
-        part[ci.base + ((long long)w.chunk * 6 + k) * ci.n + w.i0 + lane] =
-            v[k];
+      for (int k = 0; k < Pol::kOut; ++k)
+        part[ci.base + ((long long)w.chunk * Pol::kOut + k) * ci.n + w.i0 +
+             lane] = v[k];
     }
   }
 }
@@ -641,8 +903,10 @@ struct tupan_ensemble {
   // FP32C AccJerk (mode TUPAN_MODE_FP32C)
   int mode = TUPAN_MODE_FP64;
   int ntiles = 0;
-  AJ32::JRec *rec32 = nullptr;
-  double *orig = nullptr;
+  EnsNearRec *rec32 = nullptr;
+  AJ32::FarRec *frec32 = nullptr;
+  TileBox *boxes = nullptr;
+  double *orig = nullptr, *porg = nullptr;
   ClusterCtx *cctx = nullptr;
   int *tcid = nullptr;
   // launch shape (autotuner)
@@ -747,10 +1011,28 @@ void ens_enqueue_step(tupan_ensemble *e) {
            e->fields[FJY], e->fields[FJZ]}};
   TSAJ::Params tp{e->eta};
   AJ::Params ap{};
+  const bool f32c = e->mode == TUPAN_MODE_FP32C;
+  // FP32C: tile boxes and origins, then the j records, of the current state
+  auto tiles32 = [&]() {
+    hipLaunchKernelGGL(ens_tiles, dim3(blocks_for(e->ntiles)),
+                       dim3(kThreads), 0, e->st, e->ntiles, e->tcid, e->cl,
+                       e->cctx, e->active, f, e->orig, e->boxes, e->porg);
+    hipLaunchKernelGGL(ens_pack32, dim3(gb), dim3(kThreads), 0, e->st,
+                       e->ntot, f, e->cid, e->cl, e->cctx, e->active,
+                       e->orig, e->porg, e->rec32, e->frec32);
+  };
   // Tstep and the derivatives at the step's start state: one pass
-  hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st, e->ntot,
-                     f, e->cid, e->active, e->rec);
-  launch_pairs<TSAJ>(e, tp);
+  if (f32c) {
+    tiles32();
+    hipLaunchKernelGGL((ens_tiles32<Tsaj32>), dim3(e->blocks),
+                       dim3(kThreads), 0, e->st, e->rec32, e->frec32,
+                       e->boxes, e->orig, e->porg, e->items, e->nitems,
+                       e->cl, e->cctx, e->active, f, e->part, tp);
+  } else {
+    hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st, e->ntot,
+                       f, e->cid, e->active, e->rec);
+    launch_pairs<TSAJ>(e, tp);
+  }
   hipLaunchKernelGGL((ens_finish<TSAJ>), dim3(gb), dim3(kThreads), 0, e->st,
                      e->ntot, e->cid, e->cl, e->active, e->part, tsaj, tp);
   hipLaunchKernelGGL(ens_choose_dt, dim3(e->ncl), dim3(kThreads), 0, e->st,
@@ -758,18 +1040,13 @@ void ens_enqueue_step(tupan_ensemble *e) {
                      e->t_end, e->err);
   hipLaunchKernelGGL(ens_predict, dim3(gb), dim3(kThreads), 0, e->st,
                      e->ntot, f, e->cid, e->active, e->dt);
-  const bool f32c = e->mode == TUPAN_MODE_FP32C;
   for (int pass = 1; pass < 3; ++pass) {
     if (f32c) {
-      hipLaunchKernelGGL(ens_origins, dim3(blocks_for(e->ntiles)),
-                         dim3(kThreads), 0, e->st, e->ntiles, e->tcid, e->cl,
-                         e->cctx, e->active, f, e->orig);
-      hipLaunchKernelGGL(ens_pack32, dim3(gb), dim3(kThreads), 0, e->st,
-                         e->ntot, f, e->cid, e->cl, e->cctx, e->active,
-                         e->orig, e->rec32);
-      hipLaunchKernelGGL(ens_pairs32, dim3(e->blocks), dim3(kThreads), 0,
-                         e->st, e->rec32, e->items, e->nitems, e->cl,
-                         e->cctx, e->active, f, e->orig, e->part);
+      tiles32();
+      hipLaunchKernelGGL((ens_tiles32<AJ32E>), dim3(e->blocks),
+                         dim3(kThreads), 0, e->st, e->rec32, e->frec32,
+                         e->boxes, e->orig, e->porg, e->items, e->nitems,
+                         e->cl, e->cctx, e->active, f, e->part, ap);
     } else {
       hipLaunchKernelGGL(ens_pack, dim3(gb), dim3(kThreads), 0, e->st,
                          e->ntot, f, e->cid, e->active, e->rec);
@@ -857,9 +1134,15 @@ int tupan_ensemble_create(tupan_ensemble **out, int ncl, const int *sizes,
       (er = hipMalloc(&e->pairs, sizeof(double))) != hipSuccess ||
       (er = hipMalloc(&e->count, sizeof(int))) != hipSuccess ||
       (er = hipMalloc(&e->err, sizeof(int))) != hipSuccess ||
-      (er = hipMalloc(&e->rec32, sizeof(AJ32::JRec) * nt)) !=
+      (er = hipMalloc(&e->rec32, sizeof(EnsNearRec) * nt)) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->frec32, sizeof(AJ32::FarRec) * nt)) !=
           hipSuccess ||
       (er = hipMalloc(&e->orig, sizeof(double) * 4 * e->ntiles)) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->porg, sizeof(double) * 4 * e->ntiles)) !=
+          hipSuccess ||
+      (er = hipMalloc(&e->boxes, sizeof(TileBox) * e->ntiles)) !=
           hipSuccess ||
       (er = hipMalloc(&e->cctx, sizeof(ClusterCtx) * ncl)) != hipSuccess ||
       (er = hipMalloc(&e->tcid, sizeof(int) * e->ntiles)) != hipSuccess) {
@@ -894,6 +1177,7 @@ int tupan_ensemble_destroy(tupan_ensemble *e) {
   void *ptrs[] = {e->slab,   e->rec,    e->cid,   e->tc,
                   e->dt,     e->active, e->nsteps, e->pairs,
                   e->count,  e->err,    e->rec32, e->orig,
+                  e->frec32, e->porg,   e->boxes,
                   e->cctx,   e->tcid};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);

@@ -72,9 +72,9 @@ class Ensemble(object):
         self.order = order
 
     def set_mode(self, mode):
-        """'fp64' or 'fp32c' (the two corrector AccJerk sums of a step in
-        FP32C arithmetic, one i-particle per lane; the Tstep + AccJerk pass
-        at the step's start state stays FP64)."""
+        """'fp64' or 'fp32c' (the three pair passes of a step -- Tstep +
+        AccJerk at the start state, two corrector AccJerk sums -- in FP32C
+        arithmetic, one i-particle per lane; the updates stay FP64)."""
         if mode not in ("fp64", "fp32c"):
             raise ValueError("ensemble mode: 'fp64' or 'fp32c'")
         _lib.check(_lib.lib().tupan_ensemble_set_mode(self._h,
