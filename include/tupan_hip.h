@@ -284,17 +284,26 @@ int tupan_kepler_host(const double *const *ip, double dt, double *const *op);
 
 /* ---- pair-selected variants for hybrid schemes -------------------------
  * No counterpart in the reference (which applies Sakura to every pair).
- * For a splitting whose close pairs (r < rcut) are drifted by the Sakura
- * two-body Kepler solve and whose other pairs are kicked:
- *   tupan_acc_far_dev:    Acc (FP64) over the pairs with r >= rcut only;
- *   tupan_sakura_near_dev: Sakura over the pairs with r < rcut only (the
- *                          other pairs contribute nothing).
- * Arguments as tupan_acc_dev / tupan_sakura_dev; rcut >= 0. */
+ * For a splitting whose close pairs are drifted by the Sakura two-body
+ * Kepler solve and whose other pairs are kicked, the pairs are split by
+ * their separation at classification positions c = (cx, cy, cz), extra
+ * fields after the usual ones: a scheme passes positions fixed for the
+ * whole step (tupan_amd's sakura_pn: the free-drift midpoint r + v dt/2
+ * predicted at the step's start), so that every kick and drift of the
+ * step splits the pairs the same way (passing the positions themselves
+ * splits by the current separation):
+ *   tupan_acc_far_dev:    Acc (FP64) over the pairs with |c_i - c_j| >=
+ *                          rcut only; ip, jp = {m, rx, ry, rz, e2, cx, cy,
+ *                          cz};
+ *   tupan_sakura_near_dev: Sakura over the pairs with |c_i - c_j| < rcut
+ *                          only (the other pairs contribute nothing); ip,
+ *                          jp = {m, rx, ry, rz, e2, vx, vy, vz, cx, cy, cz}.
+ * Otherwise arguments as tupan_acc_dev / tupan_sakura_dev; rcut >= 0. */
 int tupan_acc_far_dev(size_t ni, const double *const *ip, size_t nj,
                       const double *const *jp, double rcut, double *const *op,
                       void *stream);
-/* tupan_acc_far_dev of a set on itself (ip = {m, rx, ry, rz, e2}, n each)
- * followed by tupan_kick_pn_half_dev(v, acc, w, tau) -- kick_pn's first
+/* tupan_acc_far_dev of a set on itself (ip = {m, rx, ry, rz, e2, cx, cy,
+ * cz}, n each) followed by tupan_kick_pn_half_dev(v, acc, w, tau) -- kick_pn's first
  * half kick, tupan/integrator/sia.py:142-156 -- in the same launches: acc
  * and v end bit-identical to the two calls.  No reference counterpart. */
 int tupan_acc_far_kick_pn_half_dev(size_t n, const double *const *ip,
@@ -447,6 +456,12 @@ int tupan_divide_dev(size_t n, int k, const double *const *src,
  * counterpart: NREG with device-resident scalars). */
 int tupan_divide_by_dev(size_t n, int k, const double *const *src,
                         double *const *dst, const double *d_s, void *stream);
+/* out = r + (v * tau) / 2, with tupan_drift_dev's operations for its half
+ * drift, r and v unchanged (no reference counterpart: the classification
+ * positions of the hybrid scheme, the free-drift midpoint of the step). */
+int tupan_half_drift_to_dev(size_t n, const double *const *r,
+                            const double *const *v, double tau,
+                            double *const *out, void *stream);
 
 /* ---- reductions ---------------------------------------------------------
  * No counterpart in the reference (it sums with numpy on the host).

@@ -320,19 +320,29 @@ def sakura_do_step(K, ps, tau, eta=None, adaptive=False):
 
 # ---- hybrid PN + Sakura scheme ---------------------------------------------
 # Not in the reference (its Sakura acts on every pair, its PN runs in SIA):
-# close pairs (r < rcut) are drifted by the Sakura two-body Kepler solve,
-# the other pairs are kicked together with the PN terms (kick_pn's w
-# scheme and flux bookkeeping):
+# close pairs are drifted by the Sakura two-body Kepler solve, the other
+# pairs are kicked together with the PN terms (kick_pn's w scheme and flux
+# bookkeeping):
 #   K(tau/2) [far Newton + PN]  D(tau) [free drift + close-pair Kepler]
 #   K(tau/2)
+# The pairs are split by their separation at the classification positions
+# cx, cy, cz: the free-drift midpoint r + (v tau) / 2 predicted at the start
+# of the step.
+
+def _class_r2(ps):
+    c = np.stack([ps["cx"], ps["cy"], ps["cz"]])
+    d = c[:, :, None] - c[:, None, :]
+    return (d * d).sum(0)
+
 
 def acc_far(ps, rcut):
-    """Acc over the pairs with r >= rcut (r^2 > 0), plain numpy."""
+    """Acc over the pairs whose classification positions are >= rcut apart
+    (r^2 > 0), plain numpy."""
     x = np.stack([ps["rx"], ps["ry"], ps["rz"]])
     r = x[:, :, None] - x[:, None, :]
     r2 = (r * r).sum(0)
     s2 = r2 + (ps["eps2"][:, None] + ps["eps2"][None, :])
-    keep = (r2 > 0) & (r2 >= rcut * rcut)
+    keep = (r2 > 0) & (_class_r2(ps) >= rcut * rcut)
     y = np.where(keep, 1.0 / np.sqrt(np.where(keep, s2, 1.0)), 0.0)
     k = ps["mass"][None, :] * y**3
     a = -(k[None] * r).sum(2)
@@ -341,12 +351,11 @@ def acc_far(ps, rcut):
 
 
 def sakura_near(K, ps, dt, flag, rcut):
-    """Sakura deltas from the pairs with r^2 < rcut^2 (self pairs included,
-    as in the full kernel), each pair through the C kernel."""
+    """Sakura deltas from the pairs whose classification positions are
+    closer than rcut (self pairs included, as in the full kernel), each pair
+    through the C kernel."""
     n = len(ps["mass"])
-    x = np.stack([ps["rx"], ps["ry"], ps["rz"]])
-    r = x[:, :, None] - x[:, None, :]
-    r2 = (r * r).sum(0)
+    r2 = _class_r2(ps)
     d = np.zeros((6, n))
     F8 = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
     for i in range(n):
@@ -375,6 +384,8 @@ def hybrid_drift(K, ps, tau, rcut, pn=False):
 
 
 def hybrid_step(K, ps, tau, rcut, pn=True):
+    for d in "xyz":
+        ps["c" + d] = ps["r" + d] + ps["v" + d] * tau / 2
     far = lambda p: acc_far(p, rcut)
     kick(K, ps, 0.5 * tau, pn, acc=far)
     hybrid_drift(K, ps, tau, rcut, pn)

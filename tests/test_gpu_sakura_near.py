@@ -11,8 +11,15 @@
   dense loop) bit for bit: the mask selects exactly the contributing pairs
   and keeps their order;
 * a cut-off of zero selects nothing: zero outputs, clean status;
+* the pairs are selected by the classification positions (cx, cy, cz),
+  not by the current ones: classification positions spread far apart
+  select no close pair (sakura_near zero, acc_far the full Acc), and
+  classification positions all equal select every pair (sakura_near the
+  full Sakura, acc_far zero);
 * acc_far_kick_pn_half (far-pair Acc with kick_pn's first half kick in the
   same launches) gives the bits of acc_far followed by kick_pn_half.
+The tests other than the classification test pass the current positions
+as classification positions.
 """
 import numpy as np
 import pytest
@@ -60,9 +67,13 @@ def _cluster(n, seed, close=6, dup=2):
     return d
 
 
-def _dev(torch, d):
-    return {f: torch.as_tensor(np.ascontiguousarray(d[f]), device="cuda")
-            for f in F8}
+def _dev(torch, d, scale=1.0):
+    """Device fields, with classification positions = scale * positions."""
+    t = {f: torch.as_tensor(np.ascontiguousarray(d[f]), device="cuda")
+         for f in F8}
+    for c in "xyz":
+        t["c" + c] = t["r" + c] if scale == 1.0 else t["r" + c] * scale
+    return t
 
 
 def _near(torch, t, dt, flag, rcut):
@@ -81,6 +92,8 @@ def test_sakura_near_matches_checker(torch, K, n, flag):
     got, status = _near(torch, _dev(torch, d), dt, flag, rcut)
     assert status == 0
     ref = {f: d[f].copy() for f in F8}
+    for c in "xyz":
+        ref["c" + c] = ref["r" + c].copy()
     oi.sakura_near(K, ref, dt, flag, rcut)
     touched = 0
     for k, f in enumerate(OUT):
@@ -118,6 +131,34 @@ def test_sakura_near_zero_cut_selects_nothing(torch):
         assert not np.any(got[k])
 
 
+def test_selection_follows_the_classification_positions(torch):
+    from tupan_amd import device
+    d = _cluster(700, 11, close=6, dup=2)
+    rcut, dt = 1e-3, 1.0 / 512
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    full_s = device.compute("sakura", _dev(torch, d), _dev(torch, d), dt, 1,
+                            status=st)
+    full_a = device.compute("acc", _dev(torch, d), _dev(torch, d))
+    # classification positions 1e6 times further apart: no close pair
+    far = _dev(torch, d, scale=1e6)
+    got, status = _near(torch, far, dt, 1, rcut)
+    assert status == 0 and not any(np.any(g) for g in got)
+    a = device.compute("acc_far", far, far, rcut)
+    for x, y in zip(a, full_a):
+        assert np.allclose(x.cpu().numpy(), y.cpu().numpy(), rtol=1e-12,
+                           atol=0)
+    # classification positions all equal: every pair is close
+    same = _dev(torch, d, scale=0.0)
+    got, status = _near(torch, same, dt, 1, rcut)
+    torch.cuda.synchronize()
+    assert status == 0 and int(st.item()) == 0
+    for k in range(6):
+        a_ = full_s[k].cpu().numpy()
+        assert np.array_equal(a_.view(np.uint64), got[k].view(np.uint64)), k
+    a = device.compute("acc_far", same, same, rcut)
+    assert not any(bool(x.any()) for x in a)
+
+
 @pytest.mark.parametrize("n", [1000, 2048, 70000])
 def test_acc_far_kick_half_is_bitwise_the_two_calls(torch, n):
     from tupan_amd import _lib
@@ -129,6 +170,7 @@ def test_acc_far_kick_half_is_bitwise_the_two_calls(torch, n):
           for _ in range(3)]
     w = [torch.as_tensor(rng.standard_normal(n), device="cuda")
          for _ in range(3)]
+    f5 = f5 + f5[1:4]  # classification positions: the positions
     ptr = lambda ts: [t.data_ptr() for t in ts]
     rcut, tau = 0.05, 1.0 / 1024
     a1 = [torch.empty(n, dtype=torch.float64, device="cuda") for _ in range(3)]

@@ -276,6 +276,8 @@ def test_hybrid_pieces_reduce_to_the_reference_kernels(K):
     ps = pn_cluster(K, n=12, seed=21)
     full = K.o.call("acc_kernel", ps, ps)
     p = oi.copy(ps)
+    for d in "xyz":  # classify by the current positions
+        p["c" + d] = p["r" + d].copy()
     oi.acc_far(p, 0.0)
     for k, d in enumerate("xyz"):
         assert np.allclose(p["a" + d], full[k], rtol=1e-13, atol=1e-15)

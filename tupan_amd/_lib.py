@@ -124,6 +124,7 @@ def lib():
         L.tupan_divide_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP,
                                        ctypes.c_double, _P]
         L.tupan_divide_by_dev.argtypes = [_SZ, ctypes.c_int, _PP, _PP, _P, _P]
+        L.tupan_half_drift_to_dev.argtypes = [_SZ, _PP, _PP, _D, _PP, _P]
         L.tupan_nreg_scalars_dev.argtypes = [_P, ctypes.c_int,
                                              ctypes.c_double, _P]
         L.tupan_nreg_step_end_dev.argtypes = [_SZ, _P, _P, _P, _P,

@@ -129,7 +129,8 @@ struct SparsePairs<Pol, decltype((void)Pol::kSparse)> {
 //
 // near_mask_kernel: lanes own j's (two 32-j tiles per wave, positions in
 // VGPRs) and walk the i-block's 64 i's, whose positions are wave-uniform
-// scalar loads; bit = r^2 < Pol::mask_cut(prm), a cut-off slightly above
+// scalar loads; bit = r^2 < Pol::mask_cut(prm) (r from the positions in
+// fields ClassX<Pol>::value..+2), a cut-off slightly above
 // the policy's own so that a different rounding of r^2 cannot drop a pair.
 // A separate kernel because the sparse policies' pair() (a Kepler solve)
 // needs several times the registers of this test and would run it at a
@@ -137,6 +138,17 @@ struct SparsePairs<Pol, decltype((void)Pol::kSparse)> {
 // `same`: the i and j sets are the same arrays, so j == i is the particle
 // paired with itself; where Pol::self_noop says that pair adds exactly
 // nothing, it gets no bit.
+// Field of the x coordinate the sparse selection tests (Pol::kClassX when
+// the policy selects by other positions than its own, else RX).
+template <class Pol, class = void>
+struct ClassX {
+  static constexpr int value = RX;
+};
+template <class Pol>
+struct ClassX<Pol, decltype((void)Pol::kClassX)> {
+  static constexpr int value = Pol::kClassX;
+};
+
 template <class Pol>
 __global__ __launch_bounds__(kBlock) void near_mask_kernel(
     Soa is, int ni, Soa js, int nj, int ntiles, typename Pol::Params prm,
@@ -148,7 +160,8 @@ __global__ __launch_bounds__(kBlock) void near_mask_kernel(
   const int j0 = (blockIdx.y * kWaves + wave) * 64;
   if (i0 >= ni || j0 >= nj) return;  // wave-uniform
   const int j = min(j0 + lane, nj - 1);
-  const double xj = js.f[RX][j], yj = js.f[RY][j], zj = js.f[RZ][j];
+  constexpr int X = ClassX<Pol>::value;
+  const double xj = js.f[X][j], yj = js.f[X + 1][j], zj = js.f[X + 2][j];
   const double cut = Pol::mask_cut(prm);
   // the lane's own index when its self pair is skipped, else -1
   const int self = same && Pol::self_noop(js, j, prm) ? j : -1;
@@ -156,9 +169,9 @@ __global__ __launch_bounds__(kBlock) void near_mask_kernel(
   bool nb = false;
 #pragma unroll 8
   for (int i = i0; i < ie; ++i) {
-    const double dx = is.f[RX][i] - xj;
-    const double dy = is.f[RY][i] - yj;
-    const double dz = is.f[RZ][i] - zj;
+    const double dx = is.f[X][i] - xj;
+    const double dy = is.f[X + 1][i] - yj;
+    const double dz = is.f[X + 2][i] - zj;
     nb |= (dx * dx + dy * dy + dz * dz < cut) & (i != self);
   }
   const uint64_t bits = __builtin_amdgcn_ballot_w64(nb && j0 + lane < nj);

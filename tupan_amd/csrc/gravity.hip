@@ -140,20 +140,38 @@ struct AccF64 : SumOut<3> {
   }
 };
 
-// Acc restricted to the pairs with r^2 >= rc2 (the far field of a
-// hybrid scheme whose close pairs are integrated by the Sakura Kepler
-// drift).  FP64; otherwise AccF64.
-struct AccFarF64 : AccF64 {
+// Acc restricted to the pairs whose classification positions (fields
+// 5-7: cx, cy, cz) are at least rcut apart: the far field of a hybrid
+// scheme whose close pairs are integrated by the Sakura Kepler drift
+// (SakuraNearF64 takes the complementary set of pairs).  The scheme passes
+// positions fixed for the whole step, so that every kick and drift of the
+// step splits the pairs the same way; passing the positions themselves
+// selects by the current separation.  FP64, AccF64's pair terms.
+struct AccFarF64 : SumOut<3> {
+  static constexpr bool kTwoLevel = false;
+  static constexpr int kFields = 8, kMaxIpl = 4, kPrefIpl = 1;
   struct Params { double rc2; };
-  __device__ static I load_i(const Soa &s, int i, const Params &) {
-    return PhiF64::load_i(s, i, NoParams{});
+  struct JRec { double m, x, y, z, e2, cx, cy, cz; };
+  struct I { double x, y, z, e2, cx, cy, cz; };
+  using A64 = F64Acc<3>;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
+            s.f[E2][j], s.f[5][j],  s.f[6][j],  s.f[7][j]};
   }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[E2][i],
+            s.f[5][i],  s.f[6][i],  s.f[7][i]};
+  }
+  __device__ static void zero(A64 &a) { zero_acc(a); }
+  __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   __device__ static void pair(const I &q, const JRec &r, A64 &a,
                               const Params &prm) {
     double rx = q.x - r.x, ry = q.y - r.y, rz = q.z - r.z;
     double r2 = rx * rx + ry * ry + rz * rz;
+    double cx = q.cx - r.cx, cy = q.cy - r.cy, cz = q.cz - r.cz;
+    double c2 = cx * cx + cy * cy + cz * cz;
     double y = rsq64(r2 + (q.e2 + r.e2));
-    y = (r2 > 0.0 && r2 >= prm.rc2) ? y : 0.0;
+    y = (r2 > 0.0 && c2 >= prm.rc2) ? y : 0.0;
     double k = r.m * y * (y * y);
     a.v[0] -= k * rx;
     a.v[1] -= k * ry;

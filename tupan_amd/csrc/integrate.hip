@@ -505,6 +505,41 @@ static int divide(size_t n, int k, const double *const *src,
   return finish("divide");
 }
 
+// out = r + (v * tau) / 2: drift_kernel's half drift, written elsewhere
+__global__ __launch_bounds__(kThreads) void half_drift_to_kernel(int n, P18 p,
+                                                                double tau) {
+#pragma clang fp contract(off)
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i < n;
+       i += gridDim.x * kThreads) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double dr = p.f[3 + d][i] * tau;
+      dr = dr / 2;
+      p.f[6 + d][i] = p.f[d][i] + dr;
+    }
+  }
+}
+
+int tupan_half_drift_to_dev(size_t n, const double *const *r,
+                            const double *const *v, double tau,
+                            double *const *out, void *stream) {
+  int rc;
+  if ((rc = check_n(n))) return rc;
+  if ((rc = check_fields((const void *const *)r, 3, "position"))) return rc;
+  if ((rc = check_fields((const void *const *)v, 3, "velocity"))) return rc;
+  if ((rc = check_fields((const void *const *)out, 3, "output"))) return rc;
+  if (n == 0) return TUPAN_OK;
+  P18 p{};
+  for (int d = 0; d < 3; ++d) {
+    p.f[d] = const_cast<double *>(r[d]);
+    p.f[3 + d] = const_cast<double *>(v[d]);
+    p.f[6 + d] = out[d];
+  }
+  hipLaunchKernelGGL(half_drift_to_kernel, dim3(grid_for(n)), dim3(kThreads),
+                     0, reinterpret_cast<hipStream_t>(stream), (int)n, p, tau);
+  return finish("half_drift_to");
+}
+
 int tupan_divide_dev(size_t n, int k, const double *const *src,
                      double *const *dst, double s, void *stream) {
   return divide(n, k, src, dst, s, nullptr, stream);

@@ -89,6 +89,54 @@ struct SakuraF64 : SumOut<6> {
   }
 };
 
+// Sakura restricted to the pairs whose classification positions (fields
+// 8-10: cx, cy, cz) are closer than rcut: the close pairs of the hybrid
+// scheme of AccFarF64, split by the same positions (fixed for the step,
+// so that both Sakura passes of a drift and both kicks agree on every
+// pair).  The zero state (a particle paired with itself) is
+// recognised from the current state, as in SakuraF64.
+struct SakuraNearF64 : SakuraF64 {
+  static constexpr int kFields = 11;
+  static constexpr int kClassX = 8;  // near_mask_kernel tests fields 8-10
+  struct JRec { double m, x, y, z, e2, vx, vy, vz, cx, cy, cz; };
+  using I = JRec;
+  __device__ static JRec pack(const Soa &s, int j, double) {
+    return {s.f[M][j],  s.f[RX][j], s.f[RY][j], s.f[RZ][j],
+            s.f[E2][j], s.f[VX][j], s.f[VY][j], s.f[VZ][j],
+            s.f[8][j],  s.f[9][j],  s.f[10][j]};
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    return pack(s, i, 0.0);
+  }
+  __device__ static double class_r2(const I &q, const JRec &r) {
+    double cx = q.cx - r.cx, cy = q.cy - r.cy, cz = q.cz - r.cz;
+    return cx * cx + cy * cy + cz * cz;
+  }
+  __device__ static bool near(const I &q, const JRec &r, const Params &prm) {
+    double dx = q.x - r.x, dy = q.y - r.y, dz = q.z - r.z;
+    double r2 = dx * dx + dy * dy + dz * dz;
+    return class_r2(q, r) < prm.rc2 &&
+           !zero_state(r2, q.vx - r.vx, q.vy - r.vy, q.vz - r.vz, q.m + r.m,
+                       prm.dt);
+  }
+  __device__ static void pair(const I &q, const JRec &r, A64 &a,
+                              const Params &prm) {
+    if (!(class_r2(q, r) < prm.rc2)) return;
+    double st[6] = {q.x - r.x,   q.y - r.y,   q.z - r.z,
+                    q.vx - r.vx, q.vy - r.vy, q.vz - r.vz};
+    const double r2 = st[0] * st[0] + st[1] * st[1] + st[2] * st[2];
+    if (zero_state(r2, st[3], st[4], st[5], q.m + r.m, prm.dt)) return;
+    const double r0[6] = {st[0], st[1], st[2], st[3], st[4], st[5]};
+    double m = q.m + r.m;
+    double e2 = q.e2 + r.e2;
+    if (!pair_evolve(prm.dt, prm.flag, m, e2, st, prm.budget) && prm.status)
+      atomicOr(prm.status, 1);
+    double w = r.m / m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.v[k] += w * (st[k] - r0[k]);
+  }
+};
+
 }  // namespace tupan
 
 using namespace tupan;
@@ -111,9 +159,9 @@ int tupan_sakura_near_dev(size_t ni, const double *const *ip, size_t nj,
   if (!(rcut >= 0.0))
     return set_error(TUPAN_ERR_ARG, "rcut must be >= 0 (got %g)", rcut);
   SakuraParams prm{dt, flag, sakura_budget(), d_status, rcut * rcut};
-  return run_engine<SakuraF64>(TUPAN_K_SAKURA, TUPAN_MODE_FP64, ni, ip, nj,
-                               jp, op, prm,
-                               reinterpret_cast<hipStream_t>(stream));
+  return run_engine<SakuraNearF64>(TUPAN_K_SAKURA, TUPAN_MODE_FP64, ni, ip,
+                                   nj, jp, op, prm,
+                                   reinterpret_cast<hipStream_t>(stream));
 }
 
 int tupan_sakura_host(size_t ni, const double *const *ip, size_t nj,

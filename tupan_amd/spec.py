@@ -23,6 +23,9 @@ from collections import namedtuple
 F5 = ("mass", "rx", "ry", "rz", "eps2")
 F8 = F5 + ("vx", "vy", "vz")
 F14 = F8 + ("ax", "ay", "az", "jx", "jy", "jz")
+# classification positions of the pair-selected kernels (positions fixed
+# for a hybrid step)
+CP = ("cx", "cy", "cz")
 
 Kernel = namedtuple("Kernel", "name kid fields outs has_mode flops")
 
@@ -42,9 +45,10 @@ KERNELS = {
     "nreg_v": Kernel("nreg_v", 10, ("mass", "vx", "vy", "vz", "ax", "ay", "az"),
                      ("mvx", "mvy", "mvz", "mk"), True, 25),
     # pair-selected variants for hybrid schemes (no reference counterpart):
-    # Acc over the pairs with r >= rcut, Sakura over the pairs with r < rcut
-    "acc_far": Kernel("acc_far", 1, F5, ("ax", "ay", "az"), False, 20),
-    "sakura_near": Kernel("sakura_near", 6, F8,
+    # Acc over the pairs whose classification positions CP are >= rcut
+    # apart, Sakura over the pairs < rcut apart
+    "acc_far": Kernel("acc_far", 1, F5 + CP, ("ax", "ay", "az"), False, 20),
+    "sakura_near": Kernel("sakura_near", 6, F8 + CP,
                           ("drx", "dry", "drz", "dvx", "dvy", "dvz"), False,
                           None),
     # reverse-mode derivatives (no reference counterpart): inputs F5 plus the
