@@ -15,8 +15,8 @@ the acceleration and the jerk, for every particle, against
 FP32C calls with nj > 4096 run over Morton-sorted particles with FP64
 recomputation of the rows whose estimated error exceeds 0.7 |result|
 (tupan_set_flag_tau); the remaining tests cover that path's edges: the
-threshold set to 0 (every row through the FP64 recomputation, past its
-1024-row chunked part), i and j different arrays, coincident particles,
+threshold set to 0 (every row through the FP64 recomputation, also past
+the rows its grid holds), i and j different arrays, coincident particles,
 sizes around the threshold, the unsorted engine, run-to-run bit identity
 and graph replay.
 """
@@ -139,18 +139,27 @@ def test_c2_16k_every_kernel_both_modes():
                 assert mx <= 1e-6, (name, q, mx)
 
 
-def test_flag_threshold_zero_recomputes_every_row_in_fp64():
-    """tau = 0: every row with an FP32 tile is recomputed in FP64 -- the
-    first 1024 in j chunks, the rest whole -- so FP32C equals the FP64
-    kernel to FP64 rounding (sums in another order)."""
-    n = 8192
+@pytest.mark.parametrize("ni,nj", [(8192, 8192), (20000, 4160)])
+def test_flag_threshold_zero_recomputes_every_row_in_fp64(ni, nj):
+    """tau = 0: every row with an FP32 tile is recomputed in FP64, so
+    FP32C equals the FP64 kernel to FP64 rounding (sums in another order).
+    8192 rows fit the recomputation grid's chunked rows (64 per block of
+    16 * recompute_groups(nj)); with nj = 4160 it holds 17408 rows, so
+    2592 of the 20000 are evaluated whole by the finishing kernel."""
     h = c2_model(16384)
-    h = {f: v[:n].copy() for f, v in h.items()}
-    ref = kernels.compute("acc_jerk", h, h, mode="fp64")
+    hi = {f: v[:ni].copy() for f, v in h.items()} if ni <= 16384 else \
+        {f: np.concatenate([v, v[:ni - 16384] * (1.0 + 1e-3)])
+         for f, v in h.items()}
+    hj = {f: v[:nj].copy() for f, v in h.items()}
+    hj = hi if (ni == nj) else hj
+    ref = kernels.compute("acc_jerk", hi, hj, mode="fp64")
     _lib.set_flag_tau(0.0)
-    got = kernels.compute("acc_jerk", h, h, mode="fp32c")
-    flagged = _lib.last_flagged()
-    assert flagged == n, flagged
+    try:
+        got = kernels.compute("acc_jerk", hi, hj, mode="fp32c")
+        flagged = _lib.last_flagged()
+    finally:
+        _lib.set_flag_tau(0.7)
+    assert flagged == ni, flagged
     for grp in ((0, 1, 2), (3, 4, 5)):
         assert rel_rows(got, ref, grp).max() <= 1e-12, grp
 
