@@ -61,11 +61,12 @@ constexpr int kSortMaxBlocks = 256;
 constexpr int kSctxVals = 14;                  // per-block context values
 // Rows whose estimated FP32 error exceeds this fraction of |result| are
 // recomputed in FP64 (the default of tupan_set_flag_tau).  The estimate
-// sqrt(sum_t |tile partial|^2) bounded the error of every row by 8.7e-7 of
-// itself (CPU emulation of these kernels' arithmetic, bit for bit with the
-// GPU's, on the Plummer models N = 16384, 65536 and 262144, seed 1), so an
-// unflagged row stays below 0.7 * 8.7e-7 = 6.1e-7 (3.7e-7 measured); 0.3-0.6%
-// of the rows are flagged at the default near field (c = 5).
+// sqrt(sum_t |tile partial|^2) bounded the error of every row by 9.2e-7 of
+// itself (CPU emulation of these kernels' arithmetic, far records included,
+// bit for bit with the GPU's, on the Plummer models N = 16384, 65536 and
+// 262144, seed 1), so an unflagged row stays below 0.7 * 9.2e-7 = 6.4e-7
+// (4.1e-7 measured); 0.6-0.7% of the rows are flagged at the default near
+// field (c = 5).
 constexpr double kFlagTau = 0.7;
 // The FP64 recomputation runs (kRecomputeRows / 64) * recompute_groups(nj)
 // blocks; recompute_shape spreads the flagged rows over them.
