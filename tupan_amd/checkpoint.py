@@ -2,9 +2,11 @@
 
 The reference restarts from a pickle of its whole Simulation object
 (tupan/simulation.py:178-185, 222-228) and dumps snapshots as HDF5 groups
-``Snapshot[_NNNNNN]/<system>/<member>`` (tupan/io/hdf5io.py:49-64).  Here a
-checkpoint is one numpy ``.npz`` file that loads with ``allow_pickle=False``
-(nothing in it executes on load), holding
+``Snapshot[_NNNNNN]/<system>/<member>``, one structured dataset per member
+type (tupan/io/hdf5io.py:49-64).  Here a checkpoint is one numpy ``.npz``
+file that loads with ``allow_pickle=False`` (nothing in it executes on
+load), holding the same snapshot naming in a per-field variant (one array
+per field of the one particle set, not one structured dataset per member):
 
   ``Snapshot[_NNNNNN]/system/<field>``  every field of the device System
                                         (state, derivatives, PN accumulators,
