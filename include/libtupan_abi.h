@@ -8,7 +8,11 @@
  * precisions (tupan/lib/cffi_backend.py:31-87, c_common.h:7-43).
  *
  * Each call is synchronous and runs on the MI355X through libtupan_hip.so
- * in that library's default mode (tupan_set_default_mode; FP64 unless set).
+ * in the library's mode (tupan_abi_set_mode below): libtupan_f64.so follows
+ * the process default (tupan_set_default_mode; FP64 unless set);
+ * libtupan_f32.so computes Phi, Acc, AccJerk, SnapCrackle and Tstep in the
+ * reference FP32 build's arithmetic (TUPAN_MODE_FP32_IEEE: its results bit
+ * for bit) and the other kernels in FP64, rounded to float.
  * The reference functions report nothing; on failure these write NaN to
  * every output and tupan_abi_last_status() returns the TUPAN_ERR_* code of
  * the last call: 0 on success, TUPAN_ERR_NOT_CONVERGED when a Sakura or
@@ -35,6 +39,12 @@ extern "C" {
 #endif
 
 int tupan_abi_last_status(void);
+/* No counterpart in the reference: the library's arithmetic mode, a
+ * TUPAN_MODE_* of tupan_hip.h or -1 (follow tupan_set_default_mode);
+ * kernels without TUPAN_MODE_FP32_IEEE run FP64 in that mode.  Returns 0,
+ * or TUPAN_ERR_ARG for an unknown mode. */
+int tupan_abi_set_mode(int mode);
+int tupan_abi_get_mode(void);
 
 /* libtupan.h:2-15 */
 void phi_kernel(const UINT ni, const REAL *_im, const REAL *_irx,

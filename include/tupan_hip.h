@@ -33,8 +33,14 @@
  *                               contraction, correctly rounded division and
  *                               sqrt, j summed in ascending order (one j
  *                               split); results equal the reference's bits
- *       (Phi, Acc, AccJerk and SnapCrackle have all three; Tstep, PNAcc and
- *       NREG X/V have no FP32C variant and run FP64 there; Sakura and
+ *       TUPAN_MODE_FP32_IEEE -- the same for the reference's FP32 build:
+ *                               each input rounded to float (exact for
+ *                               values that came from float arrays), float
+ *                               arithmetic in the reference's order, the
+ *                               float results widened exactly
+ *       (Phi, Acc, AccJerk and SnapCrackle have all four; Tstep has all but
+ *       FP32C, where it runs FP64; PNAcc and NREG X/V have FP64 and
+ *       FP64_IEEE, run FP64 for FP32C and refuse FP32_IEEE; Sakura and
  *       Kepler always run FP64 and take no mode).
  *
  * Every function returns 0 on success or a TUPAN_ERR_* code; the text of the
@@ -62,7 +68,8 @@ extern "C" {
 #define TUPAN_MODE_FP64 0
 #define TUPAN_MODE_FP32C 1
 #define TUPAN_MODE_FP64_IEEE 2
-#define TUPAN_MODE_COUNT 3
+#define TUPAN_MODE_FP32_IEEE 3
+#define TUPAN_MODE_COUNT 4
 
 /* kernel ids for tupan_set_launch */
 #define TUPAN_K_PHI 0

@@ -40,6 +40,24 @@ def test_unknown_modes_rejected(L):
         _err(L, fn(7, 4, _pp(8), 4, _pp(8), 0.1, _pp(8), NULL))
 
 
+def test_fp32_ieee_mode_where_it_exists(L):
+    """Phi, Acc, AccJerk, SnapCrackle and Tstep take TUPAN_MODE_FP32_IEEE
+    (empty work: no device); PNAcc and NREG X/V refuse it."""
+    m = _lib.MODE_FP32_IEEE
+    for name, nf, no in (("phi", 5, 1), ("acc", 5, 3), ("acc_jerk", 8, 6),
+                         ("snap_crackle", 14, 6)):
+        fn = getattr(L, "tupan_%s_dev" % name)
+        assert fn(m, 0, _pp(nf), 0, _pp(nf), _pp(no), NULL) == _lib.OK
+    assert L.tupan_tstep_dev(m, 0, _pp(8), 0, _pp(8), 0.1, _pp(2),
+                             NULL) == _lib.OK
+    inv = (ctypes.c_double * 7)(*([0.1] * 7))
+    _err(L, L.tupan_pnacc_dev(m, 0, _pp(8), 0, _pp(8), 7, inv, _pp(3),
+                              NULL))
+    for name in ("nreg_x", "nreg_v"):
+        fn = getattr(L, "tupan_%s_dev" % name)
+        _err(L, fn(m, 0, _pp(8), 0, _pp(8), 0.1, _pp(8), NULL))
+
+
 def test_null_fields_and_sizes_rejected(L):
     _err(L, L.tupan_acc_dev(0, 4, NULL, 4, NULL, NULL, NULL))
     _err(L, L.tupan_acc_dev(0, 4, _pp(5), 4, _pp(5), _pp(3), NULL))

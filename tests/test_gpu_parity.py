@@ -201,8 +201,20 @@ def test_dropin_f64_matches_c_abi(golden):
             assert np.array_equal(x, y, equal_nan=True), case
 
 
-def test_dropin_f32_at_least_as_accurate_as_reference_f32(golden):
+@pytest.mark.parametrize("abi_mode", [_lib.MODE_FP32_IEEE, _lib.MODE_FP64])
+def test_dropin_f32_at_least_as_accurate_as_reference_f32(golden, abi_mode):
+    """libtupan_f32.so against the FP64 truth: in its default mode (the
+    reference FP32 build's arithmetic where it exists, FP64 rounded to float
+    elsewhere) and with FP64 arithmetic throughout (tupan_abi_set_mode)."""
     lib = _load_dropin("32")
+    assert lib.tupan_abi_set_mode(abi_mode) == 0
+    try:
+        _f32_vs_truth(lib, golden)
+    finally:
+        assert lib.tupan_abi_set_mode(_lib.MODE_FP32_IEEE) == 0
+
+
+def _f32_vs_truth(lib, golden):
     for case in golden.cases:
         if case["prec"] != "float32":
             continue

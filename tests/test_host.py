@@ -49,10 +49,25 @@ REF_SYMBOLS = ["phi_kernel", "acc_kernel", "acc_jerk_kernel",
 def test_dropin_libraries_export_libtupan_symbols(bits):
     path = os.path.join(LIBDIR, "libtupan_f%s.so" % bits)
     lib = exported(path)
-    for s in REF_SYMBOLS + ["tupan_abi_last_status"]:
-        assert hasattr(lib, s), s
     hdr = header_symbols(os.path.join(ROOT, "include", "libtupan_abi.h"))
-    assert hdr == ["tupan_abi_last_status"]
+    assert hdr == ["tupan_abi_get_mode", "tupan_abi_last_status",
+                   "tupan_abi_set_mode"]
+    for s in REF_SYMBOLS + hdr:
+        assert hasattr(lib, s), s
+
+
+def test_dropin_library_modes_need_no_device():
+    """libtupan_f64.so follows the process default mode (-1);
+    libtupan_f32.so starts in the reference FP32 build's arithmetic."""
+    for bits, start in (("64", -1), ("32", _lib.MODE_FP32_IEEE)):
+        lib = ctypes.CDLL(os.path.join(LIBDIR, "libtupan_f%s.so" % bits))
+        assert lib.tupan_abi_get_mode() == start
+        assert lib.tupan_abi_set_mode(_lib.MODE_COUNT) == _lib.ERR_ARG
+        assert lib.tupan_abi_set_mode(-2) == _lib.ERR_ARG
+        assert lib.tupan_abi_get_mode() == start
+        for m in (-1, _lib.MODE_FP64, _lib.MODE_FP64_IEEE, start):
+            assert lib.tupan_abi_set_mode(m) == _lib.OK
+            assert lib.tupan_abi_get_mode() == m
 
 
 def test_dropin_header_declares_reference_symbols():
@@ -72,7 +87,11 @@ def test_version_and_mode_calls_need_no_device():
     assert L.tupan_set_launch(2, _lib.MODE_FP64_IEEE, 2, 1) == _lib.OK
     assert L.tupan_set_launch(2, _lib.MODE_FP64_IEEE, 0, 4) == _lib.ERR_ARG
     assert L.tupan_set_launch(2, _lib.MODE_FP64_IEEE, 0, 0) == _lib.OK
-    assert L.tupan_set_launch(2, 3, 0, 0) == _lib.ERR_ARG
+    assert L.tupan_set_launch(2, _lib.MODE_FP32_IEEE, 0, 2) == _lib.ERR_ARG
+    assert L.tupan_set_launch(2, _lib.MODE_FP32_IEEE, 1, 1) == _lib.OK
+    assert L.tupan_set_launch(2, _lib.MODE_FP32_IEEE, 0, 0) == _lib.OK
+    assert L.tupan_set_launch(2, _lib.MODE_COUNT, 0, 0) == _lib.ERR_ARG
+    assert L.tupan_set_default_mode(_lib.MODE_COUNT) == _lib.ERR_ARG
     assert L.tupan_set_default_mode(_lib.MODE_FP64_IEEE) == _lib.OK
     assert L.tupan_set_default_mode(_lib.MODE_FP64) == _lib.OK
 

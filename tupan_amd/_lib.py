@@ -26,7 +26,10 @@ NREG_SCALARS = 10
 MODE_FP64 = 0
 MODE_FP32C = 1
 MODE_FP64_IEEE = 2
-MODES = {"fp64": MODE_FP64, "fp32c": MODE_FP32C, "fp64_ieee": MODE_FP64_IEEE}
+MODE_FP32_IEEE = 3
+MODE_COUNT = 4
+MODES = {"fp64": MODE_FP64, "fp32c": MODE_FP32C, "fp64_ieee": MODE_FP64_IEEE,
+         "fp32_ieee": MODE_FP32_IEEE}
 
 OK = 0
 ERR_HIP = 1

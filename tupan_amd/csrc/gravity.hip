@@ -447,8 +447,8 @@ static int dispatch(int kid, int mode, size_t ni, const double *const *ip,
     return run_engine<P64>(kid, mode, ni, ip, nj, jp, op, {}, st);
   if (mode == TUPAN_MODE_FP32C)
     return run_engine<P32c>(kid, mode, ni, ip, nj, jp, op, {}, st);
-  if (mode == TUPAN_MODE_FP64_IEEE)
-    return run_ieee(kid, ni, ip, nj, jp, op, st);
+  if (mode == TUPAN_MODE_FP64_IEEE || mode == TUPAN_MODE_FP32_IEEE)
+    return run_ieee(kid, mode == TUPAN_MODE_FP32_IEEE, ni, ip, nj, jp, op, st);
   return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
 }
 
@@ -566,8 +566,9 @@ int tupan_snap_crackle_host(int mode, size_t ni, const double *const *ip,
 int tupan_tstep_dev(int mode, size_t ni, const double *const *ip,
                     size_t nj, const double *const *jp, double eta,
                     double *const *op, void *stream) {
-  if (mode == TUPAN_MODE_FP64_IEEE)
-    return run_ieee_tstep(ni, ip, nj, jp, eta, op, as_stream(stream));
+  if (mode == TUPAN_MODE_FP64_IEEE || mode == TUPAN_MODE_FP32_IEEE)
+    return run_ieee_tstep(mode == TUPAN_MODE_FP32_IEEE, ni, ip, nj, jp, eta,
+                          op, as_stream(stream));
   if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_FP32C)
     return set_error(TUPAN_ERR_ARG, "unknown mode %d", mode);
   TstepF64::Params prm{eta};

@@ -1,5 +1,6 @@
-// ieee_engine.hpp -- the engine of the reference-arithmetic mode
-// (TUPAN_MODE_FP64_IEEE), shared by ieee.hip, pnacc.hip and nreg.hip.
+// ieee_engine.hpp -- the engine of the reference-arithmetic modes
+// (TUPAN_MODE_FP64_IEEE, and TUPAN_MODE_FP32_IEEE: the reference's FP32
+// build), shared by ieee.hip, pnacc.hip and nreg.hip.
 //
 // The reference sums every j into one accumulator per i in ascending
 // order, which leaves one independent chain per (i, output): at N = 65536
@@ -18,37 +19,43 @@ namespace ieee {
 
 constexpr int kIeeeTile = 64;  // j records per LDS tile
 
-// A policy supplies JRec (a multiple of 16 bytes, read from LDS as 16-byte
+// A policy supplies Real (the arithmetic type: double, or float for the
+// FP32 build), JRec (a multiple of 16 bytes, read from LDS as 16-byte
 // chunks), I, pack, load_i and terms(q, r, t, prm): the kTerms per-pair
 // terms t; IeeeBase's fold and finish are the reference's sum -= t and
-// out = sum (a policy with another accumulation overrides them).
-template <int N>
+// out = sum (a policy with another accumulation overrides them).  Inputs
+// and outputs stay FP64 arrays: the FP32 policies round each input to
+// float when packing (exact for values that came from float arrays) and
+// widen their float results exactly.
+template <int N, class T = double>
 struct IeeeBase {
+  using Real = T;
   static constexpr bool kTwoLevel = false;
   static constexpr int kOut = N, kTerms = N, kAcc = N;
   struct Params {};
-  __device__ static void init(double *acc) {
+  __device__ static void init(T *acc) {
 #pragma unroll
-    for (int c = 0; c < N; ++c) acc[c] = 0.0;
+    for (int c = 0; c < N; ++c) acc[c] = 0;
   }
-  __device__ static void fold(double *acc, const double *t) {
+  __device__ static void fold(T *acc, const T *t) {
 #pragma clang fp contract(off)
 #pragma unroll
     for (int c = 0; c < N; ++c) acc[c] = acc[c] - t[c];
   }
   template <class Prm>
-  __device__ static void finish(const double *acc, Out out, int i,
-                                const Prm &) {
+  __device__ static void finish(const T *acc, Out out, int i, const Prm &) {
 #pragma unroll
-    for (int c = 0; c < N; ++c) out.f[c][i] = acc[c];
+    for (int c = 0; c < N; ++c) out.f[c][i] = (double)acc[c];
   }
 };
 
-// smoothing.h:382-390 / :405-414 -- 1/(r^2+e2), zero unless r^2 > 0
-__device__ __forceinline__ double ieee_inv_r2(double r2, double e2) {
+// smoothing.h:382-390 / :405-414 -- 1/(r^2+e2), zero unless r^2 > 0; for
+// float a correctly rounded float division (as the reference's FP32 build)
+template <class T>
+__device__ __forceinline__ T ieee_inv_r2(T r2, T e2) {
 #pragma clang fp contract(off)
-  double inv = 1 / (r2 + e2);
-  return r2 > 0 ? inv : 0.0;
+  T inv = 1 / (r2 + e2);
+  return r2 > 0 ? inv : (T)0;
 }
 
 // One wave per 64 i-particles (one per lane); blocks of 4 waves share the
@@ -69,8 +76,9 @@ __global__ __launch_bounds__(256) void ieee_engine(
   const int i0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
   const bool active = i0 < ni;  // wave-uniform
   const int i = min(i0 + lane, ni - 1);
+  using R = typename Pol::Real;
   const typename Pol::I q = Pol::load_i(is, i);
-  double acc[Pol::kAcc];
+  R acc[Pol::kAcc];
   Pol::init(acc);
 
   const int ntiles = (nj + kIeeeTile - 1) / kIeeeTile;
@@ -107,14 +115,14 @@ __global__ __launch_bounds__(256) void ieee_engine(
       const int nrec = min(kIeeeTile, nj - t * kIeeeTile);
       int k = 0;
       for (; k + U <= nrec; k += U) {
-        double tt[U][Pol::kTerms];
+        R tt[U][Pol::kTerms];
 #pragma unroll
         for (int u = 0; u < U; ++u) Pol::terms(q, L[k + u], tt[u], prm);
 #pragma unroll
         for (int u = 0; u < U; ++u) Pol::fold(acc, tt[u]);
       }
       for (; k < nrec; ++k) {
-        double t1[Pol::kTerms];
+        R t1[Pol::kTerms];
         Pol::terms(q, L[k], t1, prm);
         Pol::fold(acc, t1);
       }

@@ -46,10 +46,12 @@ int host_call(size_t ni, const double *const *ip, size_t nj,
               const double *const *jp, int nfields, double *const *op,
               int nout, const DevFn &fn);
 
-// ---- ieee.hip: TUPAN_MODE_FP64_IEEE (Phi, Acc, AccJerk, SnapCrackle, Tstep) ----
-int run_ieee(int kid, size_t ni, const double *const *ip, size_t nj,
-             const double *const *jp, double *const *op, hipStream_t st);
-int run_ieee_tstep(size_t ni, const double *const *ip, size_t nj,
+// ---- ieee.hip: TUPAN_MODE_FP64_IEEE / FP32_IEEE (Phi, Acc, AccJerk,
+// SnapCrackle, Tstep) ----
+int run_ieee(int kid, bool fp32, size_t ni, const double *const *ip,
+             size_t nj, const double *const *jp, double *const *op,
+             hipStream_t st);
+int run_ieee_tstep(bool fp32, size_t ni, const double *const *ip, size_t nj,
                    const double *const *jp, double eta, double *const *op,
                    hipStream_t st);
 

@@ -11,9 +11,11 @@
  * symbol names (tupan/lib/cffi_backend.py:31-87, tupan/lib/src/c_common.h:
  * 7-43).  Each exported function has the signature of its declaration in
  * tupan/lib/src/libtupan.h and forwards to the synchronous host-pointer
- * entry points of libtupan_hip.so with the library's default mode
- * (tupan_set_default_mode).  FP32 arrays are widened to FP64 for the device
- * and the results narrowed back.
+ * entry points of libtupan_hip.so in the library's mode (tupan_abi_set_mode:
+ * the FP64 library follows tupan_set_default_mode, the FP32 library computes
+ * in the reference FP32 build's arithmetic where that mode exists).  FP32
+ * arrays are widened to FP64 for the device (exactly) and the results
+ * narrowed back (exactly, for FP32 results).
  *
  * The reference functions return void and report nothing.  On a device or
  * argument failure these shims write NaN to every output, print the error
@@ -32,6 +34,33 @@
 static int g_status = 0;
 
 EXPORT int tupan_abi_last_status(void) { return g_status; }
+
+/* The library's arithmetic mode: -1 follows the process default
+ * (tupan_set_default_mode).  The FP32 library starts in
+ * TUPAN_MODE_FP32_IEEE, the reference's FP32 build bit for bit, for the
+ * kernels that have that mode (Phi, Acc, AccJerk, SnapCrackle, Tstep); the
+ * others run FP64 there. */
+#ifdef TUPAN_ABI_F64
+static int g_mode = -1;
+#else
+static int g_mode = TUPAN_MODE_FP32_IEEE;
+#endif
+
+EXPORT int tupan_abi_set_mode(int mode)
+{
+    if (mode < -1 || mode >= TUPAN_MODE_COUNT) return TUPAN_ERR_ARG;
+    g_mode = mode;
+    return TUPAN_OK;
+}
+
+EXPORT int tupan_abi_get_mode(void) { return g_mode; }
+
+static int kernel_mode(int has_fp32_ieee)
+{
+    int mode = g_mode < 0 ? tupan_get_default_mode() : g_mode;
+    if (mode == TUPAN_MODE_FP32_IEEE && !has_fp32_ieee) mode = TUPAN_MODE_FP64;
+    return mode;
+}
 
 static void report(int rc, const char *who, size_t ni, REAL **outs, int nout)
 {
@@ -112,7 +141,9 @@ static void forward(const char *who, int kind, mode_fn fn, size_t ni,
     if (bad) {
         rc = TUPAN_ERR_ARG;
     } else {
-        int mode = tupan_get_default_mode();
+        /* kinds 0 (Phi, Acc, AccJerk, SnapCrackle) and 1 (Tstep) have the
+         * FP32 IEEE mode */
+        int mode = kernel_mode(kind <= 1);
         switch (kind) {
         case 0: rc = fn(mode, ni, iw, nj, jw, ow); break;
         case 1: rc = tupan_tstep_host(mode, ni, iw, nj, jw, eta, ow); break;

@@ -433,8 +433,9 @@ int tupan_set_launch(int kernel_id, int mode, int ipl, int splits) {
   if (ipl != 0 && ipl != 1 && ipl != 2 && ipl != 4)
     return set_error(TUPAN_ERR_ARG, "ipl must be 0, 1, 2 or 4");
   if (splits < 0) return set_error(TUPAN_ERR_ARG, "splits must be >= 0");
-  if (mode == TUPAN_MODE_FP64_IEEE && splits > 1)
-    return set_error(TUPAN_ERR_ARG, "the IEEE mode runs one j split");
+  if ((mode == TUPAN_MODE_FP64_IEEE || mode == TUPAN_MODE_FP32_IEEE) &&
+      splits > 1)
+    return set_error(TUPAN_ERR_ARG, "the IEEE modes run one j split");
   std::lock_guard<std::mutex> lk(g_mu);
   g_override[kernel_id][mode] = LaunchShape{ipl, splits};
   return TUPAN_OK;

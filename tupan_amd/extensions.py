@@ -127,7 +127,9 @@ class HIPKernel(object):
         outs = rest[len(rest) - nout:]
         if self.key == "pnacc":
             scalars = (scalars[0], tuple(scalars[1:8]))
-        mode = self.mode or default_mode()
+        mode = self.mode or default_mode(self.prec)
+        if mode == "fp32_ieee" and self.key not in spec.FP32_IEEE:
+            mode = "fp64"   # no FP32 IEEE variant
         for x in tuple(iarr) + tuple(outs):
             if len(x) < ni:
                 raise ValueError("%s: array shorter than ni = %d"
@@ -187,17 +189,24 @@ def prepare_args(args, argtypes):
     return [argtype(arg) for (arg, argtype) in zip(args, argtypes)]
 
 
-_MODE = [os.environ.get("TUPAN_MODE", "fp64")]
+# The arithmetic of the HIP backend's calls per I/O precision: float64
+# calls default to FP64, float32 calls to the reference FP32 build's
+# arithmetic ("fp32_ieee": its results bit for bit) for the kernels that
+# have it (spec.FP32_IEEE), the others FP64 rounded to float.
+_MODE = {"float64": os.environ.get("TUPAN_MODE", "fp64"),
+         "float32": os.environ.get("TUPAN_MODE_FP32", "fp32_ieee")}
 
 
-def default_mode():
-    return _MODE[0]
+def default_mode(prec="float64"):
+    return _MODE[prec]
 
 
-def set_default_mode(mode):
+def set_default_mode(mode, prec="float64"):
     if mode not in _lib.MODES:
         raise ValueError("mode must be one of %s" % sorted(_lib.MODES))
-    _MODE[0] = mode
+    if prec not in _MODE:
+        raise ValueError("prec must be 'float64' or 'float32'")
+    _MODE[prec] = mode
 
 
 class AbstractExtension(object):

@@ -63,6 +63,9 @@ KERNELS = {
 
 KERNEL_IDS = {k: v.kid for k, v in KERNELS.items()}
 
+# kernels with the reference FP32 build's arithmetic (mode "fp32_ieee")
+FP32_IEEE = ("phi", "acc", "acc_jerk", "snap_crackle", "tstep")
+
 # reference C symbol -> kernel key
 SYMBOLS = {
     "phi_kernel": "phi",
