@@ -142,9 +142,17 @@ struct F32cBase : SumOut<N> {
   }
   template <class I, class R>
   __device__ static void pair32_farrec(const I &q, const R &r, A32 &a) {
+    pair32_farrec_u(q, r, q.e2 + r.e2, a);
+  }
+  // the same with the softening sum e2_i + e2_j given: a tile whose j's
+  // share one e2 forms it once per tile (the same float sum, so the same
+  // bits), one VALU instruction less per pair
+  template <class I, class R>
+  __device__ static void pair32_farrec_u(const I &q, const R &r, float e2ij,
+                                         A32 &a) {
     const float rx = q.px - r.x, ry = q.py - r.y, rz = q.pz - r.z;
     const float s2 = __builtin_fmaf(
-        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, e2ij)));
     Derived::body32(q, r, rx, ry, rz, s2, true, a);
   }
   // FP64 contribution of a near pair, from the caller's FP64 arrays.

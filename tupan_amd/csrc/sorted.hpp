@@ -415,7 +415,9 @@ static __global__ __launch_bounds__(256) void radix_scatter_kernel(
 // j records, one thread per sorted position p; a tile is the 32 lanes of a
 // half wave.  The last tile's spare lanes repeat its last j, which changes
 // no minimum or maximum.  kFarRec policies also get the far-tile records,
-// positions relative to the centre of the tile's box (porg).
+// positions relative to the centre of the tile's box (porg[4 t + 0..2]),
+// and porg[4 t + 3] = the records' e2 when the whole tile shares it (-1
+// otherwise, NaN included).
 template <class Pol>
 __global__ __launch_bounds__(256) void sorted_pack_kernel(
     Soa js, int nj, const int *__restrict__ perm, SoaW sj,
@@ -452,6 +454,10 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
   if constexpr (Pol::kFarRec) {
     for (int k = 0; k < 3; ++k) po[k] = 0.5 * (bx.lo[k] + bx.hi[k]);
     if (p < nj) fr[p] = Pol::far_rec(rec, po);
+    const double e2 = (double)rec.e2;
+    const bool same = tile_min(e2) == tile_max(e2) &&
+                      tile_max(e2 == e2 ? 0.0 : 1.0) == 0.0;
+    po[3] = same ? e2 : -1.0;
   }
   if ((threadIdx.x & (kTile - 1)) == 0 && p < nj) {
     const int t = p / kTile;
@@ -470,6 +476,28 @@ __global__ __launch_bounds__(256) void sorted_gather_kernel(
   if (p >= ni) return;
   const int o = perm[p];
   for (int k = 0; k < NF; ++k) si.f[k][p] = is.f[k][o];
+}
+
+// f(rec[k]) for k = 0 .. nk-1 (nk <= kTile), the next record fetched while
+// the current one is used; full tiles unrolled by two.
+template <class R, class F>
+__device__ __forceinline__ void tile_loop(const R *__restrict__ rec, int nk,
+                                          F &&f) {
+  R r = rec[0];
+  if (nk == kTile) {
+#pragma unroll 2
+    for (int k = 0; k < kTile; ++k) {
+      const R rn = rec[min(k + 1, kTile - 1)];
+      f(r);
+      r = rn;
+    }
+  } else {
+    for (int k = 0; k < nk; ++k) {
+      const R rn = rec[min(k + 1, nk - 1)];
+      f(r);
+      r = rn;
+    }
+  }
 }
 
 // pair_engine's FP32C path (kTwoLevel policies, one i per lane) over
@@ -544,21 +572,18 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
     Pol::tile_begin(q, aux, j0 / kTile);
     if ((farbits >> (tt & 63)) & 1) {
       if constexpr (Pol::kFarRec) {
-        Pol::tile_begin_far(q, porg + 4 * (j0 / kTile));
-        typename Pol::FarRec r = fr[j0];
-        if (nk == kTile) {
-#pragma unroll 2
-          for (int k = 0; k < kTile; ++k) {
-            const typename Pol::FarRec rn = fr[j0 + min(k + 1, kTile - 1)];
-            Pol::pair32_farrec(q, r, a);
-            r = rn;
-          }
+        const double *po = porg + 4 * (j0 / kTile);
+        Pol::tile_begin_far(q, po);
+        const double e2u = po[3];
+        if (e2u >= 0.0) {
+          const float e2ij = q.e2 + (float)e2u;
+          tile_loop(fr + j0, nk, [&](const typename Pol::FarRec &r) {
+            Pol::pair32_farrec_u(q, r, e2ij, a);
+          });
         } else {
-          for (int k = 0; k < nk; ++k) {
-            const typename Pol::FarRec rn = fr[j0 + min(k + 1, nk - 1)];
+          tile_loop(fr + j0, nk, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec(q, r, a);
-            r = rn;
-          }
+          });
         }
       } else {
         typename Pol::JRec r = jr[j0];
