@@ -314,7 +314,8 @@ struct ClusterCtx {
 // origin (midpoint of the tile's velocity range, tile_origin_kernel's
 // rule), the bounding box of the positions with the largest near-field
 // threshold of the tile's j's, and the box centre (the far records'
-// position origin).
+// position origin), with porg[4 g + 3] = the records' e2 when every j of
+// the tile has the same one (-1 otherwise, NaN included).
 __global__ __launch_bounds__(kThreads) void ens_tiles(
     int ntiles, const int *__restrict__ tcid, const ClusterInfo *__restrict__ cl,
     const ClusterCtx *__restrict__ cx, const int *__restrict__ active,
@@ -346,13 +347,17 @@ __global__ __launch_bounds__(kThreads) void ens_tiles(
       porg[4 * g + k] = 0.5 * (plo + phi);
     }
     double n2 = 0.0;
-    for (int j = j0; j < je; ++j)
+    const float e20 = (float)s.f[FE2][j0];
+    bool same = e20 == e20;
+    for (int j = j0; j < je; ++j) {
       n2 = fmax(n2, (double)near_threshold(s.f[FM][j], s.f[FE2][j], pc));
+      same = same && (float)s.f[FE2][j] == e20;
+    }
     bx.near2 = n2;
     bx.pad = 0.0;
     boxes[g] = bx;
     orig[4 * g + 3] = 0.0;
-    porg[4 * g + 3] = 0.0;
+    porg[4 * g + 3] = same ? (double)e20 : -1.0;
   }
 }
 
@@ -438,6 +443,10 @@ struct AJ32E {
   }
   __device__ static void pair32_farrec(const I &q, const FarRec &r, A32 &a) {
     P::pair32_farrec(q, r, a);
+  }
+  __device__ static void pair32_farrec_u(const I &q, const FarRec &r,
+                                         float e2ij, A32 &a) {
+    P::pair32_farrec_u(q, r, e2ij, a);
   }
   __device__ static void fold(const A32 &a, A64 &b) { P::fold(a, b); }
   __device__ static void values(const A64 &a, double *v) { P::values(a, v); }
@@ -529,9 +538,13 @@ struct Tsaj32 {
     return __builtin_amdgcn_ballot_w64(far);
   }
   __device__ static void pair32_farrec(const I &q, const FarRec &r, A32 &a) {
+    pair32_farrec_u(q, r, q.e2 + r.e2, a);
+  }
+  __device__ static void pair32_farrec_u(const I &q, const FarRec &r,
+                                         float e2ij, A32 &a) {
     const float rx = q.px - r.x, ry = q.py - r.y, rz = q.pz - r.z;
     const float s2 = __builtin_fmaf(
-        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, q.e2 + r.e2)));
+        rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, e2ij)));
     body32(q, r, rx, ry, rz, s2, true, a);
   }
   __device__ static void fold(const A32 &a, A64 &b) {
@@ -620,20 +633,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
       Pol::zero(a);
       if ((farbits >> (tt & 63)) & 1) {
         const typename Pol::FarRec *fr = frec + ci.off + j0;
-        typename Pol::FarRec r = fr[0];
-        if (nk == kTile) {
-#pragma unroll 2
-          for (int k = 0; k < kTile; ++k) {
-            const typename Pol::FarRec rn = fr[min(k + 1, kTile - 1)];
-            Pol::pair32_farrec(q, r, a);
-            r = rn;
-          }
+        // a tile whose j's share one e2 forms e2_i + e2_j once
+        const double e2u = porg[4 * t + 3];
+        if (e2u >= 0.0) {
+          const float e2ij = q.e2 + (float)e2u;
+          tile_loop(fr, nk, [&](const typename Pol::FarRec &r) {
+            Pol::pair32_farrec_u(q, r, e2ij, a);
+          });
         } else {
-          for (int k = 0; k < nk; ++k) {
-            const typename Pol::FarRec rn = fr[min(k + 1, nk - 1)];
+          tile_loop(fr, nk, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec(q, r, a);
-            r = rn;
-          }
+          });
         }
         Pol::fold(a, acc);
       } else {
