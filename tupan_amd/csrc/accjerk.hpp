@@ -104,6 +104,20 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
     return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i], s.f[VY][i],
             s.f[VZ][i], (float)s.f[E2][i], 0.0f, 0.0f, 0.0f, i};
   }
+  // the FP64 fields the sorted engine keeps in LDS
+  static constexpr int kI64 = 6;
+  __device__ static void save64(const I &q, double *f) {
+    f[0] = q.x, f[1] = q.y, f[2] = q.z, f[3] = q.vx, f[4] = q.vy, f[5] = q.vz;
+  }
+  __device__ static void load64(I &q, const double *f) {
+    q.x = f[0], q.y = f[1], q.z = f[2], q.vx = f[3], q.vy = f[4], q.vz = f[5];
+  }
+  // AccJerkF64's i record from q's FP64 fields and e2 (one load): the
+  // sorted engine then needs no per-lane addresses of the i fields in its
+  // loop
+  __device__ static AccJerkF64::I exact_i(const I &q, const Soa &is) {
+    return {q.x, q.y, q.z, is.f[E2][q.i], q.vx, q.vy, q.vz};
+  }
   // the i-side velocity offsets to tile t's origin
   __device__ static void tile_begin(I &q, const double *origins, int t) {
     constexpr double vs = kDebias ? kVelY2 : 1.0;

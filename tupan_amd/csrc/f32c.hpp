@@ -61,8 +61,9 @@ struct F32cBase : SumOut<N> {
   static constexpr bool kTwoLevel = true;
   static constexpr int kVelOrigins = 0;
   using Exact = P64;  // the FP64 policy (near pairs, recomputed rows)
-  // minimum waves per SIMD the sorted engine is compiled for
-  static constexpr int kWavesPerEu = 7;
+  // minimum waves per SIMD the sorted engine is compiled for (8: Phi and
+  // Acc 6-7% faster than at 7, AccJerk the same, no spill)
+  static constexpr int kWavesPerEu = 8;
   using Params = NoParams;
   using A64 = F64Acc<N>;
   using A32 = F32Acc<N>;
@@ -72,6 +73,13 @@ struct F32cBase : SumOut<N> {
   __device__ static void values(const A64 &a, double *v) { values_acc(a, v); }
   template <class I>
   __device__ static void tile_begin(I &, const double *, int) {}
+  // FP64 fields of I that the sorted engine keeps in LDS during its tile
+  // loop (none by default)
+  static constexpr int kI64 = 0;
+  template <class I>
+  __device__ static void save64(const I &, double *) {}
+  template <class I>
+  __device__ static void load64(I &, const double *) {}
 
   // Error estimate of the FP32 tile sums (used by the sorted engine): per
   // output group -- a 3-vector, or the one scalar output -- the sum over
@@ -162,8 +170,15 @@ struct F32cBase : SumOut<N> {
     float rx, ry, rz, s2;
     if (near_test(q, r, rx, ry, rz, s2)) {
       typename P64::Params p64{};
-      P64::pair(P64::load_i(is, q.i, p64), P64::pack(js, j, 0.0), a64, p64);
+      P64::pair(Derived::exact_i(q, is), P64::pack(js, j, 0.0), a64, p64);
     }
+  }
+  // the FP64 policy's i record of q (policies holding q's FP64 fields
+  // build it from them)
+  template <class I>
+  __device__ static typename P64::I exact_i(const I &q, const Soa &is) {
+    typename P64::Params p64{};
+    return P64::load_i(is, q.i, p64);
   }
 };
 

@@ -509,9 +509,9 @@ __device__ __forceinline__ void tile_loop(const R *__restrict__ rec, int nk,
 constexpr int kSWaves = 4;
 constexpr int kSBlock = 64 * kSWaves;
 
-// Pol::kWavesPerEu waves per SIMD at least: for AccJerk 7 (72 VGPRs; a few
-// values spill to scratch outside the pair loops), measured faster than the
-// 82 registers and 5 waves the compiler picks by itself.
+// Pol::kWavesPerEu waves per SIMD at least: for AccJerk 8 (63 VGPRs, no
+// spill, with its FP64 i fields in LDS), measured faster than the 82
+// registers and 5 waves the compiler picks by itself.
 template <class Pol>
 __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kWavesPerEu, 8))) void sorted_engine(
     Soa si, int ni, Soa sj, const typename Pol::JRec *__restrict__ jr, int nj,
@@ -529,6 +529,23 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
   const int i = min(i0 + lane, ni - 1);
   const typename Pol::Params prm{};
   typename Pol::I q = Pol::load_i(si, i, prm);
+  // LDS: the i's FP64 fields (Pol::kI64 of them) during the tile loop, the
+  // block's partial sums after it.  The FP64 fields are read at tile starts
+  // and in the tested loop only; kept in registers they cost the far loop
+  // its last registers at 7 waves per SIMD, and the compiler spilled them
+  // to scratch (168 MB of writes per launch at N = 65536).
+  constexpr int kI64 = Pol::kI64 > 0 ? Pol::kI64 : 1;
+  union Lds {
+    double i64[kSWaves][kI64][64];
+    double red[kSWaves - 1][kPart][64];
+  };
+  __shared__ Lds lds;
+  if constexpr (Pol::kI64 > 0) {
+    double f[kI64];
+    Pol::save64(q, f);
+#pragma unroll
+    for (int k = 0; k < kI64; ++k) lds.i64[wave][k][lane] = f[k];
+  }
   typename Pol::A64 acc;
   Pol::zero(acc);
   float sig[Pol::kGroups];
@@ -569,6 +586,15 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
     const int nk = min(kTile, je - j0);
     typename Pol::A32 a;
     Pol::zero(a);
+    if constexpr (Pol::kI64 > 0) {
+      // the barrier keeps the compiler from hoisting the loads out of the
+      // tile loop (which would keep the fields in registers again)
+      __asm__ volatile("" ::: "memory");
+      double f[kI64];
+#pragma unroll
+      for (int k = 0; k < kI64; ++k) f[k] = lds.i64[wave][k][lane];
+      Pol::load64(q, f);
+    }
     Pol::tile_begin(q, aux, j0 / kTile);
     if ((farbits >> (tt & 63)) & 1) {
       if constexpr (Pol::kFarRec) {
@@ -635,16 +661,16 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
   Pol::values(acc, v);
 #pragma unroll
   for (int g = 0; g < Pol::kGroups; ++g) v[Pol::kOut + g] = (double)sig[g];
-  __shared__ double red[kSWaves - 1][kPart][64];
+  if constexpr (Pol::kI64 > 0) __syncthreads();  // every wave done with i64
   if (wave > 0) {
 #pragma unroll
-    for (int k = 0; k < kPart; ++k) red[wave - 1][k][lane] = v[k];
+    for (int k = 0; k < kPart; ++k) lds.red[wave - 1][k][lane] = v[k];
   }
   __syncthreads();
   if (wave != 0) return;
   for (int w = 0; w < kSWaves - 1; ++w) {
 #pragma unroll
-    for (int k = 0; k < kPart; ++k) v[k] += red[w][k][lane];
+    for (int k = 0; k < kPart; ++k) v[k] += lds.red[w][k][lane];
   }
   if (i0 + lane < ni) {
 #pragma unroll
