@@ -423,7 +423,11 @@ struct AJ32E {
   using A64 = P::A64;
   using Params = AJ::Params;
   static constexpr int kOut = 6;
-  static constexpr int kWavesPerEu = 7;  // ens_tiles32's occupancy
+  static constexpr int kWavesPerEu = 8;  // ens_tiles32's occupancy
+  // the FP64 i fields ens_tiles32 keeps in LDS
+  static constexpr int kI64 = P::kI64;
+  __device__ static void save64(const I &q, double *f) { P::save64(q, f); }
+  __device__ static void load64(I &q, const double *f) { P::load64(q, f); }
   __device__ static I load_i(const Soa &s, int i, const Params &) {
     return P::load_i(s, i, NoParams{});
   }
@@ -455,8 +459,7 @@ struct AJ32E {
     float rx, ry, rz, s2;
     if (ens_near_test(q, r, rx, ry, rz, s2)) {
       AccJerkF64::Params p64{};
-      AccJerkF64::pair(AccJerkF64::load_i(s, q.i, p64),
-                       AccJerkF64::pack(s, j, 0.0), a, p64);
+      AccJerkF64::pair(P::exact_i(q, s), AccJerkF64::pack(s, j, 0.0), a, p64);
     }
   }
 };
@@ -478,6 +481,10 @@ struct Tsaj32 {
   using Params = TSAJ::Params;
   static constexpr int kOut = 8;
   static constexpr int kWavesPerEu = 7;  // ens_tiles32's occupancy
+  // the FP64 i fields ens_tiles32 keeps in LDS
+  static constexpr int kI64 = P::kI64;
+  __device__ static void save64(const I &q, double *f) { P::save64(q, f); }
+  __device__ static void load64(I &q, const double *f) { P::load64(q, f); }
   struct A32 {
     float v[8];
   };
@@ -559,9 +566,8 @@ struct Tsaj32 {
     float rx, ry, rz, s2;
     if (ens_near_test(q, r, rx, ry, rz, s2)) {
       const int i = q.i;
-      const AccJerkF64Rec qi = {s.f[M][i],  s.f[RX][i], s.f[RY][i],
-                                s.f[RZ][i], s.f[E2][i], s.f[VX][i],
-                                s.f[VY][i], s.f[VZ][i]};
+      const AccJerkF64Rec qi = {s.f[M][i], q.x,  q.y,  q.z,
+                                s.f[E2][i], q.vx, q.vy, q.vz};
       const AccJerkF64Rec rj = {s.f[M][j],  s.f[RX][j], s.f[RY][j],
                                 s.f[RZ][j], s.f[E2][j], s.f[VX][j],
                                 s.f[VY][j], s.f[VZ][j]};
@@ -588,6 +594,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
   const int wave = __builtin_amdgcn_readfirstlane(
       (int)(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)));
   const int nwaves = gridDim.x * (kThreads / 64);
+  // each wave's i's FP64 fields during its tile loops: read at tile starts
+  // and in the tested loop only, they would otherwise cost the far loop
+  // its last registers and spill
+  __shared__ double i64[kThreads / 64][Pol::kI64][64];
+  const int wslot = threadIdx.x >> 6;
   Soa soa{};
   for (int k = 0; k < 8; ++k) soa.f[k] = s.f[k];
   for (int it = wave; it < nitems; it += nwaves) {
@@ -598,6 +609,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
     int li = w.i0 + lane;
     li = li < ci.n ? li : ci.n - 1;
     typename Pol::I q = Pol::load_i(soa, ci.off + li, prm);
+    {
+      double f[Pol::kI64];
+      Pol::save64(q, f);
+#pragma unroll
+      for (int k = 0; k < Pol::kI64; ++k) i64[wslot][k][lane] = f[k];
+    }
     double wlo[3], whi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -627,6 +644,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
         farbits = __builtin_amdgcn_ballot_w64(far);
       }
       const int nk = min(kTile, w.j1 - j0);
+      {
+        // the barrier keeps the compiler from hoisting the loads out of
+        // the tile loop (which would keep the fields in registers again)
+        __asm__ volatile("" ::: "memory");
+        double f[Pol::kI64];
+#pragma unroll
+        for (int k = 0; k < Pol::kI64; ++k) f[k] = i64[wslot][k][lane];
+        Pol::load64(q, f);
+      }
       Pol::tile_begin(q, orig, t);
       Pol::tile_begin_far(q, porg + 4 * t);
       typename Pol::A32 a;
