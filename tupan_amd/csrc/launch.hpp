@@ -19,7 +19,8 @@ struct LaunchShape {
   int splits;  // j-range splits
 };
 LaunchShape choose_shape(int kernel_id, int mode, size_t ni, size_t nj,
-                         int pref_ipl, int max_ipl, int min_chunk);
+                         int pref_ipl, int max_ipl, int min_chunk,
+                         int waves_per_simd = 64);
 int device_alloc(void **p, size_t bytes, hipStream_t st);
 void device_free(void *p, hipStream_t st);
 // Reusable per-(device, stream, slot) device buffers for the asynchronous
@@ -110,7 +111,8 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   const int nseg = recompute_groups(n_j);
   const SortedLayout L = sorted_layout<Pol>(n_i, n_j, same, gj, gi, nseg);
   const LaunchShape sh =
-      choose_shape(kernel_id, mode, (size_t)n_i, (size_t)n_j, 1, 1, kTile);
+      choose_shape(kernel_id, mode, (size_t)n_i, (size_t)n_j, 1, 1, kTile,
+                   128);
   int S = sh.splits < kSWaves ? kSWaves : sh.splits;
   S = (S + kSWaves - 1) / kSWaves * kSWaves;
   int jchunk = (n_j + S - 1) / S;

@@ -80,11 +80,13 @@ static int simd_count() {
 
 // Launch shape: i-particles per lane P (the policy's preferred value unless
 // overridden) and j splits S.  The pair loop's scalar-load latency is hidden
-// by resident waves, so the grid aims at ~64 waves per SIMD over its
-// lifetime (measured best on MI355X for N = 16K..64K), never splitting j
-// below 128 records per wave.
+// by resident waves, so the grid aims at `waves_per_simd` waves per SIMD
+// over its lifetime (64 for the pair engine, measured best on MI355X for
+// N = 16K..64K; 128 for the sorted FP32C engine, whose near tiles make the
+// waves' work uneven), never splitting j below 128 records per wave.
 LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
-                         int pref_ipl, int max_ipl, int min_chunk) {
+                         int pref_ipl, int max_ipl, int min_chunk,
+                         int waves_per_simd) {
   LaunchShape o{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -92,7 +94,7 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
         mode < TUPAN_MODE_COUNT)
       o = g_override[kid][mode];
   }
-  const long target = 64L * simd_count();
+  const long target = (long)waves_per_simd * simd_count();
   // j chunks of at least 128 records, or of min_chunk (a tile, or 8 for the
   // plain FP64 loop) while that keeps the split count at 64 or below: small
   // systems need the extra waves.
