@@ -61,6 +61,7 @@ struct F32cBase : SumOut<N> {
   static constexpr bool kTwoLevel = true;
   static constexpr int kVelOrigins = 0;
   using Exact = P64;  // the FP64 policy (near pairs, recomputed rows)
+  using Sorted = Derived;  // the policy of the sorted engine's calls
   // minimum waves per SIMD the sorted engine is compiled for (8: Phi and
   // Acc 6-7% faster than at 7, AccJerk the same, no spill)
   static constexpr int kWavesPerEu = 8;

@@ -302,8 +302,119 @@ struct SnapCrackleF64 : SumOut<6> {
   }
 };
 
+// SnapCrackle over Morton-sorted particles (the sorted engine's policy of
+// SnapCrackleF32c calls): velocity, acceleration and jerk differences from
+// FP32 offsets to per-tile FP64 origins (the j offsets rounded once when
+// packed, the i offsets once per tile, as AccJerk's velocities), positions
+// from far records on far tiles and rounded FP64 differences (dsub32) on
+// the others.  A pair then forms its twelve differences with FP32
+// subtractions instead of twelve FP64 subtractions and conversions, and
+// the i's FP64 fields are needed at tile starts only (they wait in LDS).
+struct SnapCrackleF32s : F32cBase<SnapCrackleF32s, SnapCrackleF64, 6> {
+  static constexpr int kFields = 14, kMaxIpl = 1, kPrefIpl = 1;
+  static constexpr int kVelOrigins = 3;  // v, a, j: 12 doubles per tile
+  static constexpr int kOrg = 4 * kVelOrigins;
+  struct JRec {
+    double x, y, z;
+    float d[9];  // offsets of v, a, j to the tile's origins
+    float m, e2, near2;
+  };
+  struct I {
+    double x, y, z, vx, vy, vz, ax, ay, az, jx, jy, jz;
+    float e2;
+    float d[9];  // the i offsets to the current tile's origins
+    int i;
+    float px, py, pz;  // position offsets to the tile's origin (far tiles)
+  };
+  static constexpr bool kFarRec = true;
+  struct alignas(32) FarRec {
+    float x, y, z, d[9], m, e2;
+  };
+  __device__ static FarRec far_rec(const JRec &r, const double *po) {
+    FarRec f;
+    f.x = (float)(r.x - po[0]);
+    f.y = (float)(r.y - po[1]);
+    f.z = (float)(r.z - po[2]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) f.d[k] = r.d[k];
+    f.m = r.m;
+    f.e2 = r.e2;
+    return f;
+  }
+  // record of j with the tile's origins o (kOrg doubles)
+  __device__ static JRec pack_o(const Soa &s, int j, const PackCtx &c,
+                                const double *o) {
+    JRec r;
+    r.x = s.f[RX][j];
+    r.y = s.f[RY][j];
+    r.z = s.f[RZ][j];
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        r.d[3 * g + k] = (float)(s.f[VX + 3 * g + k][j] - o[4 * g + k]);
+    r.m = (float)s.f[M][j];
+    r.e2 = (float)s.f[E2][j];
+    r.near2 = near_threshold(s.f[M][j], s.f[E2][j], c);
+    return r;
+  }
+  __device__ static I load_i(const Soa &s, int i, const Params &) {
+    I q;
+    q.x = s.f[RX][i], q.y = s.f[RY][i], q.z = s.f[RZ][i];
+    q.vx = s.f[VX][i], q.vy = s.f[VY][i], q.vz = s.f[VZ][i];
+    q.ax = s.f[AX][i], q.ay = s.f[AY][i], q.az = s.f[AZ][i];
+    q.jx = s.f[JX][i], q.jy = s.f[JY][i], q.jz = s.f[JZ][i];
+    q.e2 = (float)s.f[E2][i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) q.d[k] = 0.0f;
+    q.i = i;
+    q.px = q.py = q.pz = 0.0f;
+    return q;
+  }
+  // the FP64 fields the sorted engine keeps in LDS
+  static constexpr int kI64 = 12;
+  __device__ static void save64(const I &q, double *f) {
+    f[0] = q.x, f[1] = q.y, f[2] = q.z, f[3] = q.vx, f[4] = q.vy;
+    f[5] = q.vz, f[6] = q.ax, f[7] = q.ay, f[8] = q.az, f[9] = q.jx;
+    f[10] = q.jy, f[11] = q.jz;
+  }
+  __device__ static void load64(I &q, const double *f) {
+    q.x = f[0], q.y = f[1], q.z = f[2], q.vx = f[3], q.vy = f[4];
+    q.vz = f[5], q.ax = f[6], q.ay = f[7], q.az = f[8], q.jx = f[9];
+    q.jy = f[10], q.jz = f[11];
+  }
+  __device__ static SnapCrackleF64::I exact_i(const I &q, const Soa &is) {
+    return {q.x,  q.y,  q.z,  is.f[E2][q.i], q.vx, q.vy, q.vz,
+            q.ax, q.ay, q.az, q.jx,          q.jy, q.jz};
+  }
+  // the i-side offsets to tile t's origins
+  __device__ static void tile_begin(I &q, const double *origins, int t) {
+    const double *o = origins + kOrg * t;
+    const double f[9] = {q.vx, q.vy, q.vz, q.ax, q.ay, q.az, q.jx, q.jy, q.jz};
+#pragma unroll
+    for (int g = 0; g < 3; ++g)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        q.d[3 * g + k] = (float)(f[3 * g + k] - o[4 * g + k]);
+  }
+  template <class R>  // JRec or FarRec
+  __device__ static void body32(const I &q, const R &r, float rx, float ry,
+                                float rz, float s2, bool far, A32 &a) {
+    float y = rsq32n(s2);
+    y = far ? y : 0.0f;
+    snap_crackle_pair<float>(rx, ry, rz, q.d[0] - r.d[0], q.d[1] - r.d[1],
+                             q.d[2] - r.d[2], q.d[3] - r.d[3],
+                             q.d[4] - r.d[4], q.d[5] - r.d[5],
+                             q.d[6] - r.d[6], q.d[7] - r.d[7],
+                             q.d[8] - r.d[8], y, r.m, a.v);
+  }
+};
+
 struct SnapCrackleF32c : F32cBase<SnapCrackleF32c, SnapCrackleF64, 6> {
   static constexpr int kFields = 14, kMaxIpl = 2, kPrefIpl = 1;
+  // calls with more than 4096 j's go to the sorted engine with its own
+  // policy
+  using Sorted = SnapCrackleF32s;
   // 5 waves per SIMD in the sorted engine (96 VGPRs, 116 B per lane
   // spilled, 188 B at 7 waves): 6% faster at N = 65536 than 7 waves, as
   // fast as 4

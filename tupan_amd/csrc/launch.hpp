@@ -95,6 +95,12 @@ template <class Pol>
 struct VelOrigins<Pol, decltype((void)Pol::kVelOrigins)> {
   static constexpr int value = Pol::kVelOrigins;
 };
+// doubles per tile of the sorted path's origins: 4 per origin triple
+template <class Pol>
+struct OrgStride {
+  static constexpr int value =
+      4 * (VelOrigins<Pol>::value > 0 ? VelOrigins<Pol>::value : 1);
+};
 
 }  // namespace tupan
 #include "sorted.hpp"
@@ -313,7 +319,8 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
   if constexpr (Pol::kTwoLevel) {
     // large FP32C calls: Morton-sorted particles (sorted.hpp)
     if (n_j > kSmallCtxJ && sorted_enabled())
-      return run_sorted<Pol>(kernel_id, mode, n_i, is, n_j, js, out, st);
+      return run_sorted<typename Pol::Sorted>(kernel_id, mode, n_i, is, n_j,
+                                              js, out, st);
   }
   // j chunks are whole tiles for the tiled loops (FP32C, sparse policies);
   // the plain FP64 loop takes chunks of 8 records, which tiny systems use

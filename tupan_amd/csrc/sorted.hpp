@@ -428,12 +428,17 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
   const int o = perm[min(p, nj - 1)];
   if (p < nj)
     for (int k = 0; k < Pol::kFields; ++k) sj.f[k][p] = js.f[k][o];
-  double org[4] = {0.0, 0.0, 0.0, 0.0};
+  // per tile, kOrg doubles: the midpoints of the tile's ranges of the
+  // fields VX + 3 g + k (g < VelOrigins, k < 3) at 4 g + k
+  constexpr int kOrg = OrgStride<Pol>::value;
+  double org[kOrg];
+  for (int k = 0; k < kOrg; ++k) org[k] = 0.0;
   if constexpr (VelOrigins<Pol>::value) {
-    for (int k = 0; k < 3; ++k) {
-      const double v = js.f[VX + k][o];
-      org[k] = 0.5 * (tile_min(v) + tile_max(v));
-    }
+    for (int g = 0; g < VelOrigins<Pol>::value; ++g)
+      for (int k = 0; k < 3; ++k) {
+        const double v = js.f[VX + 3 * g + k][o];
+        org[4 * g + k] = 0.5 * (tile_min(v) + tile_max(v));
+      }
   }
   const PackCtx pc{ctx[0], ctx[1], nullptr};
   typename Pol::JRec rec;
@@ -461,7 +466,7 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
   }
   if ((threadIdx.x & (kTile - 1)) == 0 && p < nj) {
     const int t = p / kTile;
-    for (int k = 0; k < 4; ++k) origins[4 * t + k] = org[k];
+    for (int k = 0; k < kOrg; ++k) origins[kOrg * t + k] = org[k];
     boxes[t] = bx;
     if constexpr (Pol::kFarRec)
       for (int k = 0; k < 4; ++k) porg[4 * t + k] = po[k];
@@ -870,7 +875,7 @@ SortedLayout sorted_layout(int ni, int nj, bool same, SortGeom gj,
   o = (o + 255) / 256 * 256;
   L.sj = take(sizeof(double) * Pol::kFields * (size_t)nj);
   L.si = same ? L.sj : take(sizeof(double) * Pol::kFields * (size_t)ni);
-  L.origins = take(sizeof(double) * 4 * (size_t)ntiles);
+  L.origins = take(sizeof(double) * OrgStride<Pol>::value * (size_t)ntiles);
   L.boxes = take(sizeof(TileBox) * (size_t)ntiles);
   L.recs = take(sizeof(typename Pol::JRec) * (size_t)nj);
   // far-tile records and the tiles' position origins (kFarRec policies)
