@@ -174,6 +174,18 @@ struct F32cBase : SumOut<N> {
       P64::pair(Derived::exact_i(q, is), P64::pack(js, j, 0.0), a64, p64);
     }
   }
+  // the same with the FP64 policy's record of j given (the sorted engine
+  // reads it with scalar loads from a packed array)
+  template <class I, class JRec>
+  __device__ static void near_pair64(const I &q, const JRec &r, A64 &a64,
+                                     const Soa &is,
+                                     const typename P64::JRec &r64) {
+    float rx, ry, rz, s2;
+    if (near_test(q, r, rx, ry, rz, s2)) {
+      typename P64::Params p64{};
+      P64::pair(Derived::exact_i(q, is), r64, a64, p64);
+    }
+  }
   // the FP64 policy's i record of q (policies holding q's FP64 fields
   // build it from them)
   template <class I>

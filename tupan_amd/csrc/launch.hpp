@@ -185,10 +185,11 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   auto *recs = (typename Pol::JRec *)at(L.recs);
   double *porg = (double *)at(L.porg);
   auto *frecs = (typename Pol::FarRec *)at(L.frecs);
+  auto *recs64 = (typename Pol::Exact::JRec *)at(L.recs64);
   hipLaunchKernelGGL((sorted_pack_kernel<Pol>), dim3((n_j + 255) / 256),
                      dim3(256), 0, st, js, n_j, perm_j, sjw,
                      (const double *)ctx, recs, origins, (TileBox *)boxes,
-                     porg, frecs);
+                     porg, frecs, recs64);
   if (!same)
     hipLaunchKernelGGL((sorted_gather_kernel<Pol::kFields>),
                        dim3((n_i + 255) / 256), dim3(256), 0, st, is, n_i,
@@ -201,7 +202,8 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
                      (const typename Pol::JRec *)recs, n_j, boxes, jchunk, part,
                      (const double *)origins,
                      (const typename Pol::FarRec *)frecs,
-                     (const double *)porg);
+                     (const double *)porg,
+                     (const typename Pol::Exact::JRec *)recs64);
   if (timed) (void)hipEventRecord(ev1, st);
   hipLaunchKernelGGL((sorted_finish_kernel<Pol>), dim3((n_i + 255) / 256),
                      dim3(256), 0, st, (const double *)part, SB, n_i, perm_i,

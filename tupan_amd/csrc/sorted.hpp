@@ -423,11 +423,14 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
     Soa js, int nj, const int *__restrict__ perm, SoaW sj,
     const double *__restrict__ ctx, typename Pol::JRec *__restrict__ jr,
     double *__restrict__ origins, TileBox *__restrict__ boxes,
-    double *__restrict__ porg, typename Pol::FarRec *__restrict__ fr) {
+    double *__restrict__ porg, typename Pol::FarRec *__restrict__ fr,
+    typename Pol::Exact::JRec *__restrict__ jr64) {
   const int p = blockIdx.x * 256 + threadIdx.x;
   const int o = perm[min(p, nj - 1)];
-  if (p < nj)
+  if (p < nj) {
     for (int k = 0; k < Pol::kFields; ++k) sj.f[k][p] = js.f[k][o];
+    jr64[p] = Pol::Exact::pack(js, o, 0.0);
+  }
   // per tile, kOrg doubles: the midpoints of the tile's ranges of the
   // fields VX + 3 g + k (g < VelOrigins, k < 3) at 4 g + k
   constexpr int kOrg = OrgStride<Pol>::value;
@@ -523,7 +526,8 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
     const TileBox *__restrict__ boxes, int jchunk,
     double *__restrict__ part, const double *__restrict__ aux,
     const typename Pol::FarRec *__restrict__ fr,
-    const double *__restrict__ porg) {
+    const double *__restrict__ porg,
+    const typename Pol::Exact::JRec *__restrict__ jr64) {
   constexpr int kPart = Pol::kPart;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -657,7 +661,7 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
       while (near) {
         const int k = __builtin_ctz(near);
         near &= near - 1u;
-        Pol::near_pair(q, jr[j0 + k], acc, si, sj, j0 + k);
+        Pol::near_pair64(q, jr[j0 + k], acc, si, jr64[j0 + k]);
       }
     }
   }
@@ -834,7 +838,7 @@ __global__ __launch_bounds__(256) void sorted_recompute_finish(
 // count.
 struct SortedLayout {
   size_t ctxpart, ctx, keys[2][2], vals[2][2], hist[2][kSortPasses], flags,
-      sj, si, origins, boxes, recs, porg, frecs, rpart, total;
+      sj, si, origins, boxes, recs, recs64, porg, frecs, rpart, total;
   size_t zero, nzero;
 };
 
@@ -878,6 +882,7 @@ SortedLayout sorted_layout(int ni, int nj, bool same, SortGeom gj,
   L.origins = take(sizeof(double) * OrgStride<Pol>::value * (size_t)ntiles);
   L.boxes = take(sizeof(TileBox) * (size_t)ntiles);
   L.recs = take(sizeof(typename Pol::JRec) * (size_t)nj);
+  L.recs64 = take(sizeof(typename Pol::Exact::JRec) * (size_t)nj);
   // far-tile records and the tiles' position origins (kFarRec policies)
   L.porg = take(Pol::kFarRec ? sizeof(double) * 4 * (size_t)ntiles : 0);
   L.frecs = take(Pol::kFarRec ? sizeof(typename Pol::FarRec) * (size_t)nj : 0);
