@@ -198,13 +198,19 @@ def test_sorted_path_other_sets_duplicates_and_reproducibility():
         assert rel_rows(unsorted, got, grp).max() <= 5e-6, grp
 
 
-def test_sorted_path_nonuniform_softening():
+@pytest.mark.parametrize("where", ["everywhere", "half"])
+def test_sorted_path_nonuniform_softening(where):
     """Per-particle softenings (the general far loop, which adds each j's
-    e2; uniform softening takes a loop with the sum precomputed) against
-    the checker, for every FP32C kernel."""
+    e2; a tile whose j's share one e2 takes a loop with the sum formed once
+    per tile) against the checker, for every FP32C kernel: random e2
+    everywhere, or only for x > 0, so that a call mixes both kinds of far
+    tiles."""
     n = 8192
     h = {f: v[:n].copy() for f, v in c2_model(16384).items()}
-    h["eps2"] = h["eps2"] * np.random.RandomState(5).uniform(0.25, 4.0, n)
+    scale = np.random.RandomState(5).uniform(0.25, 4.0, n)
+    if where == "half":
+        scale = np.where(h["rx"] > 0.0, scale, 1.0)
+    h["eps2"] = h["eps2"] * scale
     rows = np.arange(0, n, 37)
     ck = checker()
     for name, sym in (("phi", "phi_kernel"), ("acc", "acc_kernel"),
