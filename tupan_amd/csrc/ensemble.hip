@@ -663,11 +663,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
         const double e2u = porg[4 * t + 3];
         if (e2u >= 0.0) {
           const float e2ij = q.e2 + (float)e2u;
-          tile_loop(fr, nk, [&](const typename Pol::FarRec &r) {
+          tile_loop<1>(fr, nk, a, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec_u(q, r, e2ij, a);
           });
         } else {
-          tile_loop(fr, nk, [&](const typename Pol::FarRec &r) {
+          tile_loop<1>(fr, nk, a, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec(q, r, a);
           });
         }

@@ -140,6 +140,8 @@ struct F32cBase : SumOut<N> {
   // as large as the separation, so the two roundings add an error of the
   // order of dsub32's one.
   static constexpr bool kFarRec = false;
+  // far records per scalar load in full far tiles (tile_loop)
+  static constexpr int kFarGroup = 1;
   struct FarRec {
     float x, y, z, e2;
   };

@@ -98,6 +98,9 @@ struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
     return load_pos32c(s, i);
   }
   static constexpr bool kFarRec = true;
+  // far records two per scalar load: the short pair body no longer waits
+  // for its record (4 per load: slower)
+  static constexpr int kFarGroup = 2;
   using FarRec = Pos32cFar;
   __device__ static FarRec far_rec(const JRec &r, const double *po) {
     return far_pos32c(r, po);
@@ -212,6 +215,9 @@ struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
     return load_pos32c(s, i);
   }
   static constexpr bool kFarRec = true;
+  // far records two per scalar load: the short pair body no longer waits
+  // for its record (4 per load: slower)
+  static constexpr int kFarGroup = 2;
   using FarRec = Pos32cFar;
   __device__ static FarRec far_rec(const JRec &r, const double *po) {
     return far_pos32c(r, po);

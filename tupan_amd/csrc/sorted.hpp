@@ -486,13 +486,66 @@ __global__ __launch_bounds__(256) void sorted_gather_kernel(
   for (int k = 0; k < NF; ++k) si.f[k][p] = is.f[k][o];
 }
 
+// Marks the words of r as read and rewritten here, and memory as touched:
+// emits nothing, but r's pending scalar load completes before this point,
+// no load moves across it, and r's uses stay after it.
+template <class R>
+__device__ __forceinline__ void pin_sgprs(R &r) {
+  static_assert(sizeof(R) % 4 == 0, "record of 32-bit words");
+  struct W {
+    uint32_t w[sizeof(R) / 4];
+  };
+  W x = __builtin_bit_cast(W, r);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(R) / 4); ++k)
+    __asm__ volatile("" : "+s"(x.w[k]) : : "memory");
+  r = __builtin_bit_cast(R, x);
+}
+// The same for the FP32 accumulators: what follows stays after the
+// instructions that produced them.
+template <class A>
+__device__ __forceinline__ void pin_vgprs(A &a) {
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(a.v) / sizeof(a.v[0])); ++k)
+    __asm__ volatile("" : "+v"(a.v[k]));
+}
+
 // f(rec[k]) for k = 0 .. nk-1 (nk <= kTile), the next record fetched while
-// the current one is used; full tiles unrolled by two.
-template <class R, class F>
+// the current one is used; a: the FP32 accumulators that f updates.  G > 1
+// (kTile / G even): full tiles take their records G at a time (one scalar
+// load) in two buffers.  Scalar loads complete out of order, so a wait
+// covers every load in flight: each load is issued after the previous G
+// pairs and the wait for the buffer about to be used, a whole G-pair step
+// ahead of its own use.  That pays for short pair bodies (Phi, Acc: the
+// record's latency is no longer exposed every pair) and costs long ones
+// (AccJerk: the pins keep the compiler from overlapping consecutive steps).
+template <int G, class R, class A, class F>
 __device__ __forceinline__ void tile_loop(const R *__restrict__ rec, int nk,
-                                          F &&f) {
-  R r = rec[0];
-  if (nk == kTile) {
+                                          A &a, F &&f) {
+  static_assert(G == 1 || (kTile % (2 * G) == 0), "two buffers per tile");
+  if (G > 1 && nk == kTile) {
+    struct RG {
+      R r[G];
+    };
+    const RG *__restrict__ rr = reinterpret_cast<const RG *>(rec);
+    RG c0 = rr[0], c1;
+#pragma unroll 1
+    for (int k = 0; k < kTile / G; k += 2) {
+      pin_vgprs(a);
+      pin_sgprs(c0);
+      c1 = rr[k + 1];
+      pin_sgprs(c0);
+#pragma unroll
+      for (int g = 0; g < G; ++g) f(c0.r[g]);
+      pin_vgprs(a);
+      pin_sgprs(c1);
+      c0 = rr[min(k + 2, kTile / G - 1)];
+      pin_sgprs(c1);
+#pragma unroll
+      for (int g = 0; g < G; ++g) f(c1.r[g]);
+    }
+  } else if (nk == kTile) {
+    R r = rec[0];
 #pragma unroll 2
     for (int k = 0; k < kTile; ++k) {
       const R rn = rec[min(k + 1, kTile - 1)];
@@ -500,6 +553,7 @@ __device__ __forceinline__ void tile_loop(const R *__restrict__ rec, int nk,
       r = rn;
     }
   } else {
+    R r = rec[0];
     for (int k = 0; k < nk; ++k) {
       const R rn = rec[min(k + 1, nk - 1)];
       f(r);
@@ -612,11 +666,11 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
         const double e2u = po[3];
         if (e2u >= 0.0) {
           const float e2ij = q.e2 + (float)e2u;
-          tile_loop(fr + j0, nk, [&](const typename Pol::FarRec &r) {
+          tile_loop<Pol::kFarGroup>(fr + j0, nk, a, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec_u(q, r, e2ij, a);
           });
         } else {
-          tile_loop(fr + j0, nk, [&](const typename Pol::FarRec &r) {
+          tile_loop<Pol::kFarGroup>(fr + j0, nk, a, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec(q, r, a);
           });
         }
