@@ -657,6 +657,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
       Pol::tile_begin_far(q, porg + 4 * t);
       typename Pol::A32 a;
       Pol::zero(a);
+      // near pairs of a tested tile, re-evaluated in FP64 after the one
+      // fold of both kinds of tile
+      uint32_t near = 0u;
+      const typename Pol::JRec *jr = rec + ci.off + j0;
       if ((farbits >> (tt & 63)) & 1) {
         const typename Pol::FarRec *fr = frec + ci.off + j0;
         // a tile whose j's share one e2 forms e2_i + e2_j once
@@ -671,10 +675,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
             Pol::pair32_farrec(q, r, a);
           });
         }
-        Pol::fold(a, acc);
       } else {
-        uint32_t near = 0u;
-        const typename Pol::JRec *jr = rec + ci.off + j0;
         // the record of j+1 is fetched while j is computed; whole tiles
         // unrolled so that the fetch is in flight over a full pair
         typename Pol::JRec r = jr[0];
@@ -692,12 +693,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(Pol::k
             r = rn;
           }
         }
-        Pol::fold(a, acc);
-        while (near) {
-          const int k = __builtin_ctz(near);
-          near &= near - 1u;
-          Pol::near_pair(q, jr[k], acc, soa, ci.off + j0 + k, prm);
-        }
+      }
+      Pol::fold(a, acc);
+      while (near) {
+        const int k = __builtin_ctz(near);
+        near &= near - 1u;
+        Pol::near_pair(q, jr[k], acc, soa, ci.off + j0 + k, prm);
       }
     }
     if (w.i0 + lane < ci.n) {

@@ -659,6 +659,10 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
       Pol::load64(q, f);
     }
     Pol::tile_begin(q, aux, j0 / kTile);
+    // the tile's near pairs (tested tiles), re-evaluated in FP64 after the
+    // fold; one fold for both kinds of tile (the compiler then keeps the
+    // FP64 sums in place instead of copying them back every tile)
+    uint32_t near = 0u;
     if ((farbits >> (tt & 63)) & 1) {
       if constexpr (Pol::kFarRec) {
         const double *po = porg + 4 * (j0 / kTile);
@@ -691,11 +695,8 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
           }
         }
       }
-      Pol::fold(a, acc);
-      Pol::stat(a, sig);
     } else {
       typename Pol::JRec r = jr[j0];
-      uint32_t near = 0u;
       if (nk == kTile) {
 #pragma unroll 2
         for (int k = 0; k < kTile; ++k) {
@@ -710,13 +711,13 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
           r = rn;
         }
       }
-      Pol::fold(a, acc);
-      Pol::stat(a, sig);
-      while (near) {
-        const int k = __builtin_ctz(near);
-        near &= near - 1u;
-        Pol::near_pair64(q, jr[j0 + k], acc, si, jr64[j0 + k]);
-      }
+    }
+    Pol::fold(a, acc);
+    Pol::stat(a, sig);
+    while (near) {
+      const int k = __builtin_ctz(near);
+      near &= near - 1u;
+      Pol::near_pair64(q, jr[j0 + k], acc, si, jr64[j0 + k]);
     }
   }
 
