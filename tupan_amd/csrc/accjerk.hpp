@@ -65,9 +65,13 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
   // (dv_i * sum_j m_j/s^3) would not average out in the jerk; per tile it
   // is shared by 32 pairs only.
   static constexpr int kVelOrigins = 1;
+  // The velocity origins are stored multiplied by kVelY2 (kDebias), and so
+  // are the i velocities of I: the per-tile i offsets are then one FP64
+  // subtraction each, with no FP64 multiplication per tile.
+  static constexpr double kOrgScale = kDebias ? kVelY2 : 1.0;
   struct JRec { double x, y, z; float dvx, dvy, dvz, m, e2, near2; };
   struct I {
-    double x, y, z, vx, vy, vz;
+    double x, y, z, vx, vy, vz;  // vx, vy, vz times kOrgScale
     float e2, dvx, dvy, dvz;
     int i;
     float px, py, pz;  // position offsets to the tile's origin (far tiles)
@@ -82,17 +86,17 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
     return {(float)(r.x - po[0]), (float)(r.y - po[1]), (float)(r.z - po[2]),
             r.dvx, r.dvy, r.dvz, r.m, r.e2};
   }
-  // record of j with the velocity origin o (3 doubles) of its tile
+  // record of j with the (scaled) velocity origin o (3 doubles) of its tile
   __device__ static JRec pack_o(const Soa &s, int j, const PackCtx &c,
                                 const double *o) {
-    constexpr double vs = kDebias ? kVelY2 : 1.0;
+    constexpr double vs = kOrgScale;
     constexpr double ms = kDebias ? kMassY3 : 1.0;
     return {s.f[RX][j],
             s.f[RY][j],
             s.f[RZ][j],
-            (float)((s.f[VX][j] - o[0]) * vs),
-            (float)((s.f[VY][j] - o[1]) * vs),
-            (float)((s.f[VZ][j] - o[2]) * vs),
+            (float)(s.f[VX][j] * vs - o[0]),
+            (float)(s.f[VY][j] * vs - o[1]),
+            (float)(s.f[VZ][j] * vs - o[2]),
             (float)(s.f[M][j] * ms),
             (float)s.f[E2][j],
             near_threshold(s.f[M][j], s.f[E2][j], c)};
@@ -101,8 +105,10 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
     return pack_o(s, j, c, c.origins + 4 * (j / kTile));
   }
   __device__ static I load_i(const Soa &s, int i, const NoParams &) {
-    return {s.f[RX][i], s.f[RY][i], s.f[RZ][i], s.f[VX][i], s.f[VY][i],
-            s.f[VZ][i], (float)s.f[E2][i], 0.0f, 0.0f, 0.0f, i};
+    constexpr double vs = kOrgScale;
+    return {s.f[RX][i],      s.f[RY][i],      s.f[RZ][i],
+            s.f[VX][i] * vs, s.f[VY][i] * vs, s.f[VZ][i] * vs,
+            (float)s.f[E2][i], 0.0f, 0.0f, 0.0f, i};
   }
   // the FP64 fields the sorted engine keeps in LDS
   static constexpr int kI64 = 6;
@@ -112,19 +118,21 @@ struct AccJerkF32cT : F32cBase<AccJerkF32cT<kDebias>, AccJerkF64, 6> {
   __device__ static void load64(I &q, const double *f) {
     q.x = f[0], q.y = f[1], q.z = f[2], q.vx = f[3], q.vy = f[4], q.vz = f[5];
   }
-  // AccJerkF64's i record from q's FP64 fields and e2 (one load): the
-  // sorted engine then needs no per-lane addresses of the i fields in its
-  // loop
+  // AccJerkF64's i record from q's positions, and e2 and (when q's are
+  // scaled) the velocities from the caller's arrays
   __device__ static AccJerkF64::I exact_i(const I &q, const Soa &is) {
-    return {q.x, q.y, q.z, is.f[E2][q.i], q.vx, q.vy, q.vz};
+    if constexpr (kDebias)
+      return {q.x, q.y, q.z, is.f[E2][q.i], is.f[VX][q.i], is.f[VY][q.i],
+              is.f[VZ][q.i]};
+    else
+      return {q.x, q.y, q.z, is.f[E2][q.i], q.vx, q.vy, q.vz};
   }
-  // the i-side velocity offsets to tile t's origin
+  // the i-side velocity offsets to tile t's (scaled) origin
   __device__ static void tile_begin(I &q, const double *origins, int t) {
-    constexpr double vs = kDebias ? kVelY2 : 1.0;
     const double *o = origins + 4 * t;
-    q.dvx = (float)((q.vx - o[0]) * vs);
-    q.dvy = (float)((q.vy - o[1]) * vs);
-    q.dvz = (float)((q.vz - o[2]) * vs);
+    q.dvx = (float)(q.vx - o[0]);
+    q.dvy = (float)(q.vy - o[1]);
+    q.dvz = (float)(q.vz - o[2]);
   }
   template <class R>  // JRec or FarRec
   __device__ static void body32(const I &q, const R &r, float rx, float ry,

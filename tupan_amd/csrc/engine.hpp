@@ -452,10 +452,12 @@ static __global__ void ctx_final_kernel(const double *__restrict__ part,
 }
 
 // Per-tile origin of the fields `f[0..2]` (the midpoint of their range over
-// the tile's j's), for policies with kVelOrigins.  One thread per tile.
+// the tile's j's, times `scale`), for policies with kVelOrigins.  One
+// thread per tile.
 static __global__ __launch_bounds__(256) void tile_origin_kernel(
     const double *__restrict__ f0, const double *__restrict__ f1,
-    const double *__restrict__ f2, int nj, double *__restrict__ origins) {
+    const double *__restrict__ f2, int nj, double scale,
+    double *__restrict__ origins) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t * kTile >= nj) return;
   const double *f[3] = {f0, f1, f2};
@@ -466,7 +468,7 @@ static __global__ __launch_bounds__(256) void tile_origin_kernel(
       lo = fmin(lo, f[k][j]);
       hi = fmax(hi, f[k][j]);
     }
-    origins[4 * t + k] = 0.5 * (lo + hi);
+    origins[4 * t + k] = 0.5 * (lo + hi) * scale;
   }
   origins[4 * t + 3] = 0.0;
 }

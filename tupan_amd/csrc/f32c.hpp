@@ -60,6 +60,8 @@ template <class Derived, class P64, int N>
 struct F32cBase : SumOut<N> {
   static constexpr bool kTwoLevel = true;
   static constexpr int kVelOrigins = 0;
+  // factor of the stored per-tile velocity origins (midpoints times it)
+  static constexpr double kOrgScale = 1.0;
   using Exact = P64;  // the FP64 policy (near pairs, recomputed rows)
   using Sorted = Derived;  // the policy of the sorted engine's calls
   // minimum waves per SIMD the sorted engine is compiled for (8: Phi and

@@ -95,6 +95,15 @@ template <class Pol>
 struct VelOrigins<Pol, decltype((void)Pol::kVelOrigins)> {
   static constexpr int value = Pol::kVelOrigins;
 };
+// factor of the stored velocity origins (1 unless the policy says)
+template <class Pol, class = void>
+struct OrgScale {
+  static constexpr double value = 1.0;
+};
+template <class Pol>
+struct OrgScale<Pol, decltype((void)Pol::kOrgScale)> {
+  static constexpr double value = Pol::kOrgScale;
+};
 // doubles per tile of the sorted path's origins: 4 per origin triple
 template <class Pol>
 struct OrgStride {
@@ -273,8 +282,8 @@ __global__ __launch_bounds__(256) void small_pack_kernel(
           lo = fmin(lo, v[j]);
           hi = fmax(hi, v[j]);
         }
-        org[4 * tt + k] = 0.5 * (lo + hi);
-        ctx[2 + 4 * tt + k] = 0.5 * (lo + hi);
+        org[4 * tt + k] = 0.5 * (lo + hi) * OrgScale<Pol>::value;
+        ctx[2 + 4 * tt + k] = 0.5 * (lo + hi) * OrgScale<Pol>::value;
       }
     }
   }
@@ -412,7 +421,8 @@ int run_engine(int kernel_id, int mode, size_t ni, const double *const *ip,
       if (VelOrigins<Pol>::value)
         hipLaunchKernelGGL(tile_origin_kernel,
                            dim3((unsigned)((ntiles + 255) / 256)), dim3(256),
-                           0, st, js.f[VX], js.f[VY], js.f[VZ], n_j, ctx + 2);
+                           0, st, js.f[VX], js.f[VY], js.f[VZ], n_j,
+                           OrgScale<Pol>::value, ctx + 2);
     }
     if (!direct)
       hipLaunchKernelGGL((pack_kernel<Pol>), dim3((n_j + 255) / 256),

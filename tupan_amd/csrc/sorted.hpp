@@ -432,7 +432,8 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
     jr64[p] = Pol::Exact::pack(js, o, 0.0);
   }
   // per tile, kOrg doubles: the midpoints of the tile's ranges of the
-  // fields VX + 3 g + k (g < VelOrigins, k < 3) at 4 g + k
+  // fields VX + 3 g + k (g < VelOrigins, k < 3) at 4 g + k, times the
+  // policy's origin scale
   constexpr int kOrg = OrgStride<Pol>::value;
   double org[kOrg];
   for (int k = 0; k < kOrg; ++k) org[k] = 0.0;
@@ -440,7 +441,8 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
     for (int g = 0; g < VelOrigins<Pol>::value; ++g)
       for (int k = 0; k < 3; ++k) {
         const double v = js.f[VX + 3 * g + k][o];
-        org[4 * g + k] = 0.5 * (tile_min(v) + tile_max(v));
+        org[4 * g + k] =
+            0.5 * (tile_min(v) + tile_max(v)) * OrgScale<Pol>::value;
       }
   }
   const PackCtx pc{ctx[0], ctx[1], nullptr};
