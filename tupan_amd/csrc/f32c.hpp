@@ -144,6 +144,15 @@ struct F32cBase : SumOut<N> {
   static constexpr bool kFarRec = false;
   // far records per scalar load in full far tiles (tile_loop)
   static constexpr int kFarGroup = 1;
+  // kFar16 policies (a pair body that reads only the position and the
+  // mass of j): 16-byte far records {x, y, z, m} for the far tiles whose
+  // j's share one e2, stored after the FarRec array, kFar16Group per
+  // scalar load
+  static constexpr bool kFar16 = false;
+  static constexpr int kFar16Group = 4;
+  struct alignas(16) FarRec16 {
+    float x, y, z, m;
+  };
   struct FarRec {
     float x, y, z, e2;
   };

@@ -99,8 +99,10 @@ struct PhiF32c : F32cBase<PhiF32c, PhiF64, 1> {
   }
   static constexpr bool kFarRec = true;
   // far records two per scalar load: the short pair body no longer waits
-  // for its record (4 per load: slower)
+  // for its record (4 per load: slower); on tiles that share one e2 the
+  // 16-byte records, four per load
   static constexpr int kFarGroup = 2;
+  static constexpr bool kFar16 = true;
   using FarRec = Pos32cFar;
   __device__ static FarRec far_rec(const JRec &r, const double *po) {
     return far_pos32c(r, po);
@@ -216,8 +218,10 @@ struct AccF32c : F32cBase<AccF32c, AccF64, 3> {
   }
   static constexpr bool kFarRec = true;
   // far records two per scalar load: the short pair body no longer waits
-  // for its record (4 per load: slower)
+  // for its record (4 per load: slower); on tiles that share one e2 the
+  // 16-byte records, four per load
   static constexpr int kFarGroup = 2;
+  static constexpr bool kFar16 = true;
   using FarRec = Pos32cFar;
   __device__ static FarRec far_rec(const JRec &r, const double *po) {
     return far_pos32c(r, po);

@@ -463,7 +463,12 @@ __global__ __launch_bounds__(256) void sorted_pack_kernel(
   double po[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (Pol::kFarRec) {
     for (int k = 0; k < 3; ++k) po[k] = 0.5 * (bx.lo[k] + bx.hi[k]);
-    if (p < nj) fr[p] = Pol::far_rec(rec, po);
+    const typename Pol::FarRec f = Pol::far_rec(rec, po);
+    if (p < nj) fr[p] = f;
+    if constexpr (Pol::kFar16)
+      if (p < nj)
+        reinterpret_cast<typename Pol::FarRec16 *>(fr + nj)[p] = {f.x, f.y,
+                                                                  f.z, f.m};
     const double e2 = (double)rec.e2;
     const bool same = tile_min(e2) == tile_max(e2) &&
                       tile_max(e2 == e2 ? 0.0 : 1.0) == 0.0;
@@ -672,9 +677,18 @@ __global__ __launch_bounds__(kSBlock) __attribute__((amdgpu_waves_per_eu(Pol::kW
         const double e2u = po[3];
         if (e2u >= 0.0) {
           const float e2ij = q.e2 + (float)e2u;
-          tile_loop<Pol::kFarGroup>(fr + j0, nk, a, [&](const typename Pol::FarRec &r) {
-            Pol::pair32_farrec_u(q, r, e2ij, a);
-          });
+          if constexpr (Pol::kFar16) {
+            const auto *f16 =
+                reinterpret_cast<const typename Pol::FarRec16 *>(fr + nj);
+            tile_loop<Pol::kFar16Group>(
+                f16 + j0, nk, a, [&](const typename Pol::FarRec16 &r) {
+                  Pol::pair32_farrec_u(q, r, e2ij, a);
+                });
+          } else {
+            tile_loop<Pol::kFarGroup>(fr + j0, nk, a, [&](const typename Pol::FarRec &r) {
+              Pol::pair32_farrec_u(q, r, e2ij, a);
+            });
+          }
         } else {
           tile_loop<Pol::kFarGroup>(fr + j0, nk, a, [&](const typename Pol::FarRec &r) {
             Pol::pair32_farrec(q, r, a);
@@ -942,7 +956,11 @@ SortedLayout sorted_layout(int ni, int nj, bool same, SortGeom gj,
   L.recs64 = take(sizeof(typename Pol::Exact::JRec) * (size_t)nj);
   // far-tile records and the tiles' position origins (kFarRec policies)
   L.porg = take(Pol::kFarRec ? sizeof(double) * 4 * (size_t)ntiles : 0);
-  L.frecs = take(Pol::kFarRec ? sizeof(typename Pol::FarRec) * (size_t)nj : 0);
+  L.frecs = take(Pol::kFarRec ? (sizeof(typename Pol::FarRec) +
+                                 (Pol::kFar16 ? sizeof(typename Pol::FarRec16)
+                                              : 0)) *
+                                    (size_t)nj
+                              : 0);
   L.rpart = take(sizeof(double) * (size_t)kRecomputeRows * nseg * Pol::kOut);
   // nseg: chunk groups of the FP64 recomputation
   L.total = o;
