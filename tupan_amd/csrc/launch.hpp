@@ -20,7 +20,7 @@ struct LaunchShape {
 };
 LaunchShape choose_shape(int kernel_id, int mode, size_t ni, size_t nj,
                          int pref_ipl, int max_ipl, int min_chunk,
-                         int waves_per_simd = 64);
+                         int waves_per_simd = 64, int max_splits = 0);
 int device_alloc(void **p, size_t bytes, hipStream_t st);
 void device_free(void *p, hipStream_t st);
 // Reusable per-(device, stream, slot) device buffers for the asynchronous
@@ -125,9 +125,14 @@ int run_sorted(int kernel_id, int mode, int n_i, const Soa &is, int n_j,
   const SortGeom gj = sort_geom(n_j), gi = sort_geom(n_i);
   const int nseg = recompute_groups(n_j);
   const SortedLayout L = sorted_layout<Pol>(n_i, n_j, same, gj, gi, nseg);
+  // 128 waves per SIMD over the grid's lifetime, in at most 128 j chunks:
+  // the best or within 1% of it for Phi, Acc and AccJerk at N = 16K-64K in
+  // a sweep of 16-512 chunks (at N = 32768 the 256 chunks that the wave
+  // target alone gives cost Phi 8% and AccJerk 4%: shorter tile loops for
+  // the same per-wave set-up and block-row partials)
   const LaunchShape sh =
       choose_shape(kernel_id, mode, (size_t)n_i, (size_t)n_j, 1, 1, kTile,
-                   128);
+                   128, 128);
   int S = sh.splits < kSWaves ? kSWaves : sh.splits;
   S = (S + kSWaves - 1) / kSWaves * kSWaves;
   int jchunk = (n_j + S - 1) / S;

@@ -86,7 +86,7 @@ static int simd_count() {
 // waves' work uneven), never splitting j below 128 records per wave.
 LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
                          int pref_ipl, int max_ipl, int min_chunk,
-                         int waves_per_simd) {
+                         int waves_per_simd, int max_splits) {
   LaunchShape o{0, 0};
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -107,6 +107,7 @@ LaunchShape choose_shape(int kid, int mode, size_t ni, size_t nj,
   const long waves_i = (long)((ni + 64 * P - 1) / (64 * P));
   long S = o.splits > 0 ? o.splits : (target + waves_i - 1) / waves_i;
   if (S > max_split && o.splits <= 0) S = max_split;
+  if (max_splits > 0 && S > max_splits && o.splits <= 0) S = max_splits;
   if (S < 1) S = 1;
   if (nj > 0 && S > (long)nj) S = (long)nj;
   return LaunchShape{P, (int)S};
